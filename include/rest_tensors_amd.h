@@ -1,0 +1,188 @@
+/*
+ * rest_tensors_amd.h — extended C ABI of librest_tensors_amd.so.
+ *
+ * The reference has no status channel, 32-bit dimensions and host-only
+ * buffers (src/external_libs/mod.rs:6-190).  This API adds, for the RI hot
+ * path: int status codes with a thread-local message (rt_last_error), 64-bit
+ * sizes (ri3ao at nbas=1024, naux=4096 holds 2^32 doubles), device-resident
+ * buffers, explicit streams and a precision switch.
+ *
+ * All "_dev" entry points take device pointers and enqueue work on `stream`
+ * (NULL = the null stream); they return once the work is enqueued unless the
+ * comment says otherwise.  Layouts are column-major like the reference's
+ * MatrixFull / RIFull (ri.rs:18-32: element (i,j,k) at i + n0*j + n0*n1*k).
+ */
+#ifndef REST_TENSORS_AMD_H
+#define REST_TENSORS_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define RT_OK 0
+#define RT_ERR_ARG 1      /* bad shape / argument (the reference panics)   */
+#define RT_ERR_HIP 2      /* HIP runtime failure                           */
+#define RT_ERR_NOMEM 3    /* device allocation failed                      */
+#define RT_ERR_NOTCONV 4  /* iterative solver did not converge             */
+#define RT_ERR_NUMERIC 5  /* numerical breakdown (e.g. linear dependence)  */
+
+/* arithmetic of the GEMM-shaped stages (ao2mo, RI-K) */
+#define RT_PREC_FP64 0        /* FP64 MFMA                                 */
+#define RT_PREC_SPLIT_FP32 1  /* hi/lo FP32 split, 3 FP32-MFMA products    */
+#define RT_PREC_BF16X3 2      /* hi/lo BF16 split, 3 BF16-MFMA products    */
+#define RT_PREC_OZAKI_BF16 3  /* exponent-aligned BF16 slices, exact FP32
+                                 partial sums, FP64 accumulation           */
+
+/* ---- runtime plumbing -------------------------------------------------- */
+const char* rt_last_error(void);
+int rt_version(void);
+int rt_device_count(int* count);
+int rt_set_device(int device);
+int rt_device_name(char* buf, int buflen);
+int rt_malloc(void** dptr, size_t bytes);
+int rt_free(void* dptr);
+int rt_memcpy_h2d(void* dst, const void* src, size_t bytes);
+int rt_memcpy_d2h(void* dst, const void* src, size_t bytes);
+int rt_memcpy_d2d(void* dst, const void* src, size_t bytes);
+int rt_memset(void* dst, int value, size_t bytes);
+int rt_synchronize(void);
+int rt_stream_create(void** stream);
+int rt_stream_destroy(void* stream);
+int rt_stream_synchronize(void* stream);
+int rt_event_create(void** event);
+int rt_event_destroy(void* event);
+int rt_event_record(void* event, void* stream);
+int rt_event_elapsed_ms(void* start, void* stop, float* ms);
+
+/* Per-kernel timing: when enabled, the library brackets each launch of its
+ * named kernels with HIP events on the launch stream; rt_kernel_timer_get
+ * synchronizes and returns the summed duration and launch count of every
+ * kernel whose name starts with `prefix`. */
+int rt_kernel_timer_enable(int on);
+int rt_kernel_timer_reset(void);
+int rt_kernel_timer_get(const char* prefix, double* total_ms, int64_t* count);
+
+/* ---- seeded synthetic inputs ------------------------------------------ */
+/* ri3ao[i,j,P] = ri3ao[j,i,P] = z * exp(-|i-j|/decay), z ~ N(0,1) from a
+ * SplitMix64 counter hash of (seed, P, min(i,j), max(i,j)) and Box-Muller. */
+int rt_gen_ri3ao_dev(double* ri3ao, int64_t nbas, int64_t naux, uint64_t seed,
+                     double decay, void* stream);
+/* out[k] = N(0,1) draw for counter k, same hash as above (for C, eps, ...) */
+int rt_gen_normal_dev(double* out, int64_t n, uint64_t seed, void* stream);
+
+/* ---- dense FP64 GEMM on device (A6: _dgemm / general_dgemm_f) ---------- */
+/* C = alpha*op(A)*op(B) + beta*C, column-major, op in {'N','T'}.
+ * Replaces _dgemm_full (matrix_blas_lapack.rs:180-252). */
+int rt_dgemm_dev(char opa, char opb, int64_t m, int64_t n, int64_t k,
+                 double alpha, const double* A, int64_t lda, const double* B,
+                 int64_t ldb, double beta, double* C, int64_t ldc,
+                 void* stream);
+/* batched: batch b uses A + b*stride_a, B + b*stride_b, C + b*stride_c */
+int rt_dgemm_strided_batched_dev(char opa, char opb, int64_t m, int64_t n,
+                                 int64_t k, double alpha, const double* A,
+                                 int64_t lda, int64_t stride_a,
+                                 const double* B, int64_t ldb,
+                                 int64_t stride_b, double beta, double* C,
+                                 int64_t ldc, int64_t stride_c, int64_t batch,
+                                 void* stream);
+/* Sub-block GEMM with the reference's argument meaning (general_dgemm_f,
+ * external_libs/mod.rs:60-83): blocks given by 0-based starts and lengths in
+ * full column-major matrices of the stated sizes; shape mismatch -> RT_ERR_ARG
+ * (the reference's _dgemm panics, matrix_blas_lapack.rs:132-164). */
+int rt_general_dgemm_dev(const double* A, int64_t rows_a, int64_t cols_a,
+                         int64_t start_row_a, int64_t len_row_a,
+                         int64_t start_col_a, int64_t len_col_a, char opa,
+                         const double* B, int64_t rows_b, int64_t cols_b,
+                         int64_t start_row_b, int64_t len_row_b,
+                         int64_t start_col_b, int64_t len_col_b, char opb,
+                         double* C, int64_t rows_c, int64_t cols_c,
+                         int64_t start_row_c, int64_t len_row_c,
+                         int64_t start_col_c, int64_t len_col_c, double alpha,
+                         double beta, void* stream);
+
+/* ---- RIFull ao2mo (A3/A4) ---------------------------------------------- */
+/* Reference layout: ri3mo[P + naux*i + naux*ns*j] = (C^T A_P C)[i,j].
+ * ri3ao [nbas,nbas,naux], C [nbas,ns].  Replaces RIFull::ao2mo
+ * (ri.rs:356-358, 382-408) -> ri_ao2mo_f (restmatr.f90:158-194). */
+int rt_ao2mo_dev(const double* C, const double* ri3ao, double* ri3mo,
+                 int64_t ns, int64_t nbas, int64_t naux, void* stream);
+/* Occupied-row transform: B[m + nocc*i + nocc*nmo*P] =
+ *   sum_{mu,nu} C[mu,m] ri3ao[mu,nu,P] C[nu,i],  m < nocc, i < nmo.
+ * C [nbas,nmo]; the first nocc columns are the occupied orbitals.
+ * precision: RT_PREC_*.  The half-transformed intermediate lives in library
+ * scratch that grows on demand and is kept until the process ends. */
+int rt_ao2mo_occ_dev(const double* C, int64_t nbas, int64_t nmo, int64_t nocc,
+                     const double* ri3ao, int64_t naux, double* B,
+                     int precision, void* stream);
+
+/* ---- density, RI-J, RI-K (A6, A13, A14) -------------------------------- */
+/* D = C_occ C_occ^T, C_occ = first nocc columns of C [nbas, *] (ld = nbas);
+ * _dgemm_full(C_occ,'N',C_occ,'T') of matrix_blas_lapack.rs:180-252. */
+int rt_density_dev(const double* C, int64_t nbas, int64_t nocc, double* D,
+                   void* stream);
+/* gamma_P = sum_{kl} ri3ao[k,l,P] D[k,l];  J[i,j] = sum_P ri3ao[i,j,P] gamma_P;
+ * e_j (host pointer, may be NULL) = 1/2 sum_ij D_ij J_ij = 1/2 sum_P gamma_P^2.
+ * ri3ao must be symmetric in its first two indices (it is (mu nu|P)); only
+ * its upper triangle is read.  gamma: device [naux] (may be NULL).
+ * Synchronizes `stream` when e_j != NULL. */
+int rt_rij_dev(const double* ri3ao, int64_t nbas, int64_t naux,
+               const double* D, double* J, double* gamma, double* e_j,
+               void* stream);
+/* K[i,j] = sum_P sum_{m<nocc} B[m,i,P] B[m,j,P]  (B from rt_ao2mo_occ_dev);
+ * e_x (host, may be NULL) = -1/2 sum_{n<nocc} K[n,n].  K is [nmo,nmo] full
+ * (upper computed, mirrored).  precision: RT_PREC_*.
+ * Synchronizes `stream` when e_x != NULL. */
+int rt_rik_dev(const double* B, int64_t nocc, int64_t nmo, int64_t naux,
+               double* K, double* e_x, int precision, void* stream);
+/* E_x by the second route: -1/2 sum_P sum_{m,n<nocc} B[m,n,P]^2, FP64
+ * accumulation (host result; synchronizes). */
+int rt_exchange_energy_dev(const double* B, int64_t nocc, int64_t nmo,
+                           int64_t naux, double* e_x, void* stream);
+
+/* ---- packed-upper MatrixUpper (A8/A9) ---------------------------------- */
+/* Packed index of (i,j), i<=j: j*(j+1)/2 + i (index.rs:209-233).
+ * unpack: full[n,n] from packed[len]; len must be triangular (else
+ * RT_ERR_ARG, the reference returns None, matrixupper.rs:330-373). */
+int rt_upper_unpack_dev(const double* packed, int64_t packed_len,
+                        double* full, void* stream);
+/* pack: packed[n(n+1)/2] from the upper triangle of full[n,n]
+ * (MatrixFull::to_matrixupper, matrixfull.rs:638-646). */
+int rt_upper_pack_dev(const double* full, int64_t n, double* packed,
+                      void* stream);
+/* RIFull [nao,nao,naux] -> MatrixFull [nao(nao+1)/2, naux] upper-triangle
+ * pack per aux slice (rifull_to_matfull_symm, ri.rs:297-306, 330-341). */
+int rt_rifull_to_matfull_symm_dev(const double* ri, int64_t nao, int64_t naux,
+                                  double* out, void* stream);
+/* integer dimension of a packed length (exact isqrt; -1 if not triangular) */
+int64_t rt_upper_dim(int64_t packed_len);
+
+/* ---- device block movers (copy_mm/mr/rm/rr on device buffers) ---------- */
+int rt_copy_mm_dev(int64_t x_len, int64_t y_len, const double* f,
+                   int64_t f_x, int64_t f_y, int64_t f_xs, int64_t f_ys,
+                   double* t, int64_t t_x, int64_t t_y, int64_t t_xs,
+                   int64_t t_ys, void* stream);
+int rt_copy_mr_dev(int64_t x_len, int64_t y_len, const double* f,
+                   int64_t f_x, int64_t f_y, int64_t f_xs, int64_t f_ys,
+                   double* t, int64_t t_x, int64_t t_y, int64_t t_z,
+                   int64_t t_xs, int64_t t_ys, int64_t t_x3, int mod,
+                   void* stream);
+int rt_copy_rm_dev(int64_t x_len, int64_t y_len, const double* f,
+                   int64_t f_x, int64_t f_y, int64_t f_z, int64_t f_xs,
+                   int64_t f_ys, int64_t f_x3, int mod, double* t,
+                   int64_t t_x, int64_t t_y, int64_t t_xs, int64_t t_ys,
+                   void* stream);
+int rt_copy_rr_dev(int64_t x_len, int64_t y_len, int64_t z_len,
+                   const double* f, int64_t f_x, int64_t f_y, int64_t f_z,
+                   int64_t f_xs, int64_t f_ys, int64_t f_zs, double* t,
+                   int64_t t_x, int64_t t_y, int64_t t_z, int64_t t_xs,
+                   int64_t t_ys, int64_t t_zs, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* REST_TENSORS_AMD_H */

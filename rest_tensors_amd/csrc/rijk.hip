@@ -1,0 +1,254 @@
+// RI-J (A13) and RI-K / E_x (A14).
+//
+// RI-J is two HBM-bound streams over the upper triangle of ri3ao
+// (ri3ao is (mu nu|P), symmetric in mu,nu):
+//   gamma_P = sum_{k<=l} w_kl A[k,l,P] D[k,l]      (w = 2 off-diagonal, 1 on it)
+//   J[i,j]  = sum_P A[i,j,P] gamma_P               (i <= j, then mirrored)
+// Algorithmic traffic: 8*naux*nbas(nbas+1) bytes (two upper-triangle passes).
+//
+// RI-K is an MFMA-bound batch-reducing GEMM:
+//   K = sum_P B_P^T B_P,  B_P = ri3mo(occ, all, P) (nocc x nmo, contiguous)
+// computed on upper tiles only, summed over aux chunks in a fixed order.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "common.hpp"
+#include "gemm.hpp"
+#include "precision.hpp"
+#include "timer.hpp"
+
+namespace rta {
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// One block per aux slice P; wave w walks columns j = w, w+4, ...; lanes walk
+// rows 0..j of that column.  Fixed order -> deterministic.
+__global__ __launch_bounds__(256) void rij_gamma_kernel(
+    const double* __restrict__ A, const double* __restrict__ D, int64_t nbas,
+    double* __restrict__ gamma) {
+  __shared__ double red[4];
+  const int64_t P = blockIdx.x;
+  const double* Ap = A + P * nbas * nbas;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  double acc = 0.0;
+  for (int64_t j = wave; j < nbas; j += 4) {
+    const double* col = Ap + j * nbas;
+    const double* dcol = D + j * nbas;
+    for (int64_t i = lane; i <= j; i += 64) {
+      double w = (i == j) ? 1.0 : 2.0;
+      acc += w * col[i] * dcol[i];
+    }
+  }
+  acc = wave_sum(acc);
+  if (lane == 0) red[wave] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) gamma[P] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+// Block = (column j, 64-row segment s of rows 0..j); its 4 waves take every
+// 4th aux slice and are combined through LDS.  Writes J[i,j] and J[j,i].
+__global__ __launch_bounds__(256) void rij_j_kernel(
+    const double* __restrict__ A, const double* __restrict__ gamma,
+    int64_t nbas, int64_t naux, double* __restrict__ J) {
+  __shared__ double red[4][64];
+  const int64_t j = blockIdx.x;
+  const int64_t i0 = (int64_t)blockIdx.y * 64;
+  if (i0 > j) return;  // segment entirely below the diagonal
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int64_t i = i0 + lane;
+  const int64_t n2 = nbas * nbas;
+  double acc = 0.0;
+  if (i <= j) {
+    const double* p = A + i + j * nbas;
+    int64_t P = wave;
+    for (; P + 12 < naux; P += 16) {
+      double a0 = p[P * n2], a1 = p[(P + 4) * n2], a2 = p[(P + 8) * n2],
+             a3 = p[(P + 12) * n2];
+      acc += a0 * gamma[P];
+      acc += a1 * gamma[P + 4];
+      acc += a2 * gamma[P + 8];
+      acc += a3 * gamma[P + 12];
+    }
+    for (; P < naux; P += 4) acc += p[P * n2] * gamma[P];
+  }
+  red[wave][lane] = acc;
+  __syncthreads();
+  if (wave == 0 && i <= j) {
+    double v = (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
+    J[i + j * nbas] = v;
+    if (i != j) J[j + i * nbas] = v;
+  }
+}
+
+// out[0] = sum_{n<nocc} K[n,n]  (single block, fixed order)
+__global__ __launch_bounds__(256) void diag_sum_kernel(
+    const double* __restrict__ K, int64_t ld, int64_t n,
+    double* __restrict__ out) {
+  __shared__ double red[4];
+  double acc = 0.0;
+  for (int64_t k = threadIdx.x; k < n; k += 256) acc += K[k + k * ld];
+  acc = wave_sum(acc);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) out[0] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+// partial[P] = sum_{m,n<nocc} B[m,n,P]^2 over the occ-occ block of slice P
+__global__ __launch_bounds__(256) void occ_sumsq_kernel(
+    const double* __restrict__ B, int64_t nocc, int64_t nmo,
+    double* __restrict__ partial) {
+  __shared__ double red[4];
+  const int64_t P = blockIdx.x;
+  const double* b = B + P * nocc * nmo;  // occ-occ block is the first nocc^2
+  double acc = 0.0;
+  for (int64_t e = threadIdx.x; e < nocc * nocc; e += 256) acc += b[e] * b[e];
+  acc = wave_sum(acc);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) partial[P] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+// Chunk size over P for the K reduction: enough chunks that the upper-tile x
+// chunk grid covers the 256 CUs several times.
+static int64_t rik_bchunk(int64_t nmo, int64_t naux, int tile) {
+  int64_t t = (nmo + tile - 1) / tile;
+  int64_t ntiles = t * (t + 1) / 2;
+  int64_t want = std::max<int64_t>(1, (4 * 256 + ntiles - 1) / ntiles);
+  want = std::min<int64_t>(want, naux);
+  return (naux + want - 1) / want;
+}
+
+int rik_fp64(const double* B, int64_t nocc, int64_t nmo, int64_t naux,
+             double* K, hipStream_t s) {
+  const int tile = 128;
+  int64_t bchunk = rik_bchunk(nmo, naux, tile);
+  int64_t nchunks = (naux + bchunk - 1) / bchunk;
+  double* part = nullptr;
+  RTA_TRY(scratch_get(SCRATCH_GEMM_PARTIAL,
+                      sizeof(double) * std::max<int64_t>(nchunks, 1) * nmo * nmo,
+                      (void**)&part));
+  GemmDesc d;
+  d.ta = true;  // op(A)(i,m) = B_P[m + i*nocc]
+  d.tb = false; // op(B)(m,j) = B_P[m + j*nocc]
+  d.M = nmo;
+  d.N = nmo;
+  d.K = nocc;
+  d.A = B;
+  d.lda = nocc;
+  d.sA = nocc * nmo;
+  d.B = B;
+  d.ldb = nocc;
+  d.sB = nocc * nmo;
+  d.alpha = 1.0;
+  d.batch = naux;
+  return gemm_reduce_sym(d, bchunk, part, K, nmo, s, "rik_mfma", tile);
+}
+
+}  // namespace rta
+
+using namespace rta;
+
+extern "C" int rt_rij_dev(const double* ri3ao, int64_t nbas, int64_t naux,
+                          const double* D, double* J, double* gamma,
+                          double* e_j, void* stream) {
+  RTA_CHECK(nbas >= 0 && naux >= 0, "negative dimension");
+  RTA_CHECK(nbas < (int64_t(1) << 31) / 64 && naux < (int64_t(1) << 31),
+            "dimension too large");
+  hipStream_t s = (hipStream_t)stream;
+  if (nbas == 0) {
+    if (e_j) *e_j = 0.0;
+    return RT_OK;
+  }
+  RTA_CHECK(D && J, "NULL buffer");
+  RTA_CHECK(naux == 0 || ri3ao, "ri3ao is NULL");
+  double* g = gamma;
+  if (!g && naux > 0)
+    RTA_TRY(scratch_get(SCRATCH_JK, sizeof(double) * naux, (void**)&g));
+  if (naux > 0) {
+    KernelTimer kt("rij_gamma", s);
+    hipLaunchKernelGGL(rij_gamma_kernel, dim3((unsigned)naux), dim3(256), 0, s,
+                       ri3ao, D, nbas, g);
+    RTA_HIP(hipGetLastError());
+  }
+  {
+    KernelTimer kt("rij_j", s);
+    hipLaunchKernelGGL(rij_j_kernel,
+                       dim3((unsigned)nbas, (unsigned)((nbas + 63) / 64)),
+                       dim3(256), 0, s, ri3ao, g, nbas, naux, J);
+    RTA_HIP(hipGetLastError());
+  }
+  if (e_j) {
+    std::vector<double> h(naux);
+    if (naux > 0) {
+      RTA_HIP(hipMemcpyAsync(h.data(), g, sizeof(double) * naux,
+                             hipMemcpyDeviceToHost, s));
+      RTA_HIP(hipStreamSynchronize(s));
+    }
+    double acc = 0.0;
+    for (int64_t P = 0; P < naux; ++P) acc += h[P] * h[P];
+    *e_j = 0.5 * acc;
+  }
+  return RT_OK;
+}
+
+extern "C" int rt_rik_dev(const double* B, int64_t nocc, int64_t nmo,
+                          int64_t naux, double* K, double* e_x, int precision,
+                          void* stream) {
+  RTA_CHECK(nocc >= 0 && nmo >= 0 && naux >= 0, "negative dimension");
+  RTA_CHECK(nocc <= nmo, "nocc > nmo");
+  hipStream_t s = (hipStream_t)stream;
+  if (nmo == 0) {
+    if (e_x) *e_x = 0.0;
+    return RT_OK;
+  }
+  RTA_CHECK(K, "K is NULL");
+  RTA_CHECK(B || nocc == 0 || naux == 0, "B is NULL");
+  if (precision == RT_PREC_FP64)
+    RTA_TRY(rik_fp64(B, nocc, nmo, naux, K, s));
+  else
+    RTA_TRY(rik_lowp(B, nocc, nmo, naux, K, precision, s));
+  if (e_x) {
+    double* dsum = nullptr;
+    RTA_TRY(scratch_get(SCRATCH_JK, sizeof(double), (void**)&dsum));
+    hipLaunchKernelGGL(diag_sum_kernel, dim3(1), dim3(256), 0, s, K, nmo, nocc,
+                       dsum);
+    RTA_HIP(hipGetLastError());
+    double h = 0.0;
+    RTA_HIP(hipMemcpyAsync(&h, dsum, sizeof(double), hipMemcpyDeviceToHost, s));
+    RTA_HIP(hipStreamSynchronize(s));
+    *e_x = -0.5 * h;
+  }
+  return RT_OK;
+}
+
+extern "C" int rt_exchange_energy_dev(const double* B, int64_t nocc,
+                                      int64_t nmo, int64_t naux, double* e_x,
+                                      void* stream) {
+  RTA_CHECK(nocc >= 0 && nmo >= 0 && naux >= 0 && nocc <= nmo,
+            "bad dimensions");
+  RTA_CHECK(e_x, "e_x is NULL");
+  hipStream_t s = (hipStream_t)stream;
+  if (nocc == 0 || naux == 0) {
+    *e_x = 0.0;
+    return RT_OK;
+  }
+  double* part = nullptr;
+  RTA_TRY(scratch_get(SCRATCH_JK, sizeof(double) * naux, (void**)&part));
+  hipLaunchKernelGGL(occ_sumsq_kernel, dim3((unsigned)naux), dim3(256), 0, s,
+                     B, nocc, nmo, part);
+  RTA_HIP(hipGetLastError());
+  std::vector<double> h(naux);
+  RTA_HIP(hipMemcpyAsync(h.data(), part, sizeof(double) * naux,
+                         hipMemcpyDeviceToHost, s));
+  RTA_HIP(hipStreamSynchronize(s));
+  double acc = 0.0;
+  for (int64_t P = 0; P < naux; ++P) acc += h[P];
+  *e_x = -0.5 * acc;
+  return RT_OK;
+}

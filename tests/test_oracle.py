@@ -1,0 +1,234 @@
+"""Pins the CPU oracle (oracle/ri_oracle.py) before it is trusted as the checker.
+
+Each oracle function is checked against at least one of: golden vectors of the
+reference's own compiled Fortran (tests/golden/ref_vectors.npz), doc values of
+the reference, closed forms, invariants, or brute force on tiny inputs.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import ri_oracle as O
+
+
+def relf(a, b):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    nb = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (nb if nb > 0 else 1.0)
+
+
+# ---- ao2mo ---------------------------------------------------------------------
+@pytest.mark.parametrize("case", ["ao2mo_a", "ao2mo_b", "ao2mo_c", "ao2mo_ns"])
+def test_ao2mo_matches_compiled_reference(golden, case):
+    Cm, A, R = golden[f"{case}_C"], golden[f"{case}_A"], golden[f"{case}_R"]
+    out = O.ri_ao2mo_ref(Cm, A)
+    assert out.shape == R.shape
+    assert relf(out, R) < 1e-13
+    assert relf(O.ri_ao2mo_v01(Cm, A), R) < 1e-13
+
+
+def test_ao2mo_bench_closed_form(golden):
+    # benches/bench_tensors.rs:5-10: ri3fn == 2.0 [10,10,20], C == 1.0
+    assert np.all(golden["ao2mo_bench_R"] == 200.0)
+    out = O.ri_ao2mo_ref(np.ones((10, 10), order="F"), np.full((10, 10, 20), 2.0, order="F"))
+    assert out.shape == (20, 10, 10) and np.all(out == 200.0)
+
+
+def test_ao2mo_identity_is_permutation():
+    A = O.gen_ri3ao(9, 4, seed=3)
+    out = O.ri_ao2mo_ref(np.eye(9, order="F"), A)
+    for P in range(4):
+        assert np.array_equal(out[P], A[:, :, P])
+    # flat layout: element (P,i,j) at P + naux*i + naux*ns*j
+    flat = out.reshape(-1, order="F")
+    assert flat[2 + 4 * 5 + 4 * 9 * 7] == A[5, 7, 2]
+
+
+def test_ao2mo_brute_force():
+    rng = np.random.default_rng(1)
+    nbas, ns, naux = 5, 3, 3
+    A = rng.standard_normal((nbas, nbas, naux))
+    Cm = rng.standard_normal((nbas, ns))
+    out = O.ri_ao2mo_ref(Cm, A)
+    for P in range(naux):
+        for i in range(ns):
+            for j in range(ns):
+                ref = math.fsum(Cm[mu, i] * A[mu, nu, P] * Cm[nu, j]
+                                for mu in range(nbas) for nu in range(nbas))
+                assert abs(out[P, i, j] - ref) < 1e-13 * max(1.0, abs(ref))
+
+
+def test_ao2mo_orthogonal_invariance():
+    A = O.gen_ri3ao(24, 6, seed=5)
+    Cm = O.gen_orthonormal(24, seed=6)
+    out = O.ri_ao2mo_ref(Cm, A)
+    for P in range(6):
+        assert abs(np.linalg.norm(out[P]) - np.linalg.norm(A[:, :, P])) < 1e-12
+        assert abs(np.trace(out[P]) - np.trace(A[:, :, P])) < 1e-12
+        assert np.allclose(out[P], out[P].T, atol=1e-13)
+
+
+def test_ao2mo_occ_is_occupied_rows_of_reference():
+    A = O.gen_ri3ao(20, 7, seed=8)
+    Cm = O.gen_orthonormal(20, seed=9)
+    full = O.ri_ao2mo_ref(Cm, A)          # [naux, ns, ns]
+    B = O.ao2mo_occ(Cm, A, nocc=4)        # [nocc, nmo, naux]
+    assert B.shape == (4, 20, 7)
+    assert relf(B, np.transpose(full, (1, 2, 0))[:4]) < 1e-13
+
+
+# ---- inputs --------------------------------------------------------------------
+def test_generators():
+    z = O.gen_normal(200000, seed=11)
+    assert abs(z.mean()) < 0.01 and abs(z.var() - 1.0) < 0.02
+    A = O.gen_ri3ao(16, 3, seed=12)
+    assert np.array_equal(A, A.transpose(1, 0, 2))
+    # envelope: |A[i,j]| / exp(-|i-j|/16) is a normal draw
+    Q = O.gen_orthonormal(32, seed=13)
+    assert np.abs(Q.T @ Q - np.eye(32)).max() < 1e-14
+    G = O.gen_normal(32 * 32, seed=13).reshape((32, 32), order="F")
+    R = Q.T @ G
+    assert np.all(np.diag(R) > 0) and np.abs(np.tril(R, -1)).max() < 1e-12
+
+
+def test_splitmix64_known_values():
+    # SplitMix64 (Vigna) output sequence for state 0: first two outputs
+    assert int(O.splitmix64(np.uint64(0))) == 0xE220A8397B1DCDAF
+    assert int(O.splitmix64(np.uint64(0x9E3779B97F4A7C15))) == 0x6E789E6AA1B965F4
+
+
+# ---- density / J / K -------------------------------------------------------------
+def test_density_projector():
+    Cm = O.gen_orthonormal(30, seed=21)
+    D = O.density(Cm, 7)
+    assert np.allclose(D @ D, D, atol=1e-13)
+    assert abs(np.trace(D) - 7) < 1e-12
+    i, j = 3, 11
+    assert abs(D[i, j] - sum(Cm[i, m] * Cm[j, m] for m in range(7))) < 1e-15
+
+
+def test_rij_brute_force_and_energy():
+    A = O.gen_ri3ao(6, 4, seed=31)
+    Cm = O.gen_orthonormal(6, seed=32)
+    D = O.density(Cm, 2)
+    gamma, J, e_j = O.rij(A, D)
+    for P in range(4):
+        ref = math.fsum(A[k, l, P] * D[k, l] for k in range(6) for l in range(6))
+        assert abs(gamma[P] - ref) < 1e-14
+    for i in range(6):
+        for j in range(6):
+            ref = math.fsum(A[i, j, P] * gamma[P] for P in range(4))
+            assert abs(J[i, j] - ref) < 1e-14
+    assert abs(e_j - 0.5 * np.sum(gamma ** 2)) < 1e-13
+    assert np.allclose(J, J.T, atol=1e-15)
+
+
+def test_rij_identity_reduction():
+    A = O.gen_ri3ao(8, 5, seed=33)
+    D = O.density(np.eye(8, order="F"), 3)     # diag(1,1,1,0,...)
+    gamma, _, _ = O.rij(A, D)
+    assert np.allclose(gamma, A[0, 0] + A[1, 1] + A[2, 2], atol=1e-15)
+
+
+def test_rik_brute_force_and_two_routes():
+    A = O.gen_ri3ao(7, 5, seed=41)
+    Cm = O.gen_orthonormal(7, seed=42)
+    B = O.ao2mo_occ(Cm, A, 3)
+    K, e_x, e_x2 = O.rik(B)
+    for i in range(7):
+        for j in range(7):
+            ref = math.fsum(B[m, i, P] * B[m, j, P] for m in range(3) for P in range(5))
+            assert abs(K[i, j] - ref) < 1e-14
+    assert abs(e_x - e_x2) < 1e-13
+    assert np.allclose(K, K.T, atol=1e-15)
+    assert np.linalg.eigvalsh(K).min() > -1e-12
+
+
+def test_rik_single_occupied_outer_product():
+    A = O.gen_ri3ao(5, 4, seed=43)
+    Cm = O.gen_orthonormal(5, seed=44)
+    B = O.ao2mo_occ(Cm, A, 1)
+    K, _, _ = O.rik(B)
+    ref = sum(np.outer(B[0, :, P], B[0, :, P]) for P in range(4))
+    assert relf(K, ref) < 1e-15
+
+
+# ---- packed upper ----------------------------------------------------------------
+def test_upper_doc_order():
+    # matrix/mod.rs:437-452: 4x4 matrix 1..16, upper order
+    m = np.arange(1, 17).reshape((4, 4), order="F")
+    assert list(O.upper_pack(m)) == [1, 5, 6, 9, 10, 11, 13, 14, 15, 16]
+
+
+def test_upper_index_map_exhaustive():
+    for n in range(0, 41):
+        m = np.arange(n * n, dtype=float).reshape((n, n), order="F")
+        p = O.upper_pack(m)
+        assert len(p) == n * (n + 1) // 2
+        for j in range(n):
+            for i in range(j + 1):
+                assert p[j * (j + 1) // 2 + i] == m[i, j]
+        full = O.upper_unpack(p)
+        up = np.triu(m)
+        assert np.array_equal(full, up + np.triu(m, 1).T)
+        assert O.upper_dim(len(p)) == n
+    assert O.upper_unpack(np.zeros(5)) is None
+    assert O.upper_dim(524800) == 1024 and O.upper_dim(524801) == -1
+
+
+def test_rifull_to_matfull_symm():
+    ri = O.gen_ri3ao(6, 3, seed=51)
+    out = O.rifull_to_matfull_symm(ri)
+    assert out.shape == (21, 3)
+    assert out[5 * 6 // 2 + 2, 1] == ri[2, 5, 1]
+
+
+# ---- sub-block GEMM and copies -----------------------------------------------------
+def test_general_dgemm_golden(golden):
+    for ci in range(int(golden["gemm_ncases"])):
+        p = f"gemm{ci}_"
+        ra = [tuple(r) for r in golden[p + "ra"]]
+        rb = [tuple(r) for r in golden[p + "rb"]]
+        rc = [tuple(r) for r in golden[p + "rc"]]
+        opa, opb = (chr(int(c)) for c in golden[p + "ops"])
+        alpha, beta = golden[p + "ab"]
+        out = O.general_dgemm(golden[p + "A"], ra, opa, golden[p + "B"], rb, opb,
+                              golden[p + "C"], rc, alpha, beta)
+        assert relf(out, golden[p + "out"]) < 1e-14, ci
+    # doc values (matrix_blas_lapack.rs:91-119), exact
+    assert list(golden["gemm0_out"][1:3, 0:2].reshape(-1, order="F")) == [88, 101, 127, 146]
+    assert list(golden["gemm1_out"][1, 0:2]) == [74, 107]
+    assert list(golden["gemm2_out"][0:2, 0]) == [59, 74]
+
+
+def test_copies_golden(golden):
+    F, T = golden["cmm_F"], golden["cmm_T"]
+    xl, yl, fxs, fys, txs, tys = golden["cmm_args"]
+    assert np.array_equal(O.copy_mm(xl, yl, F, fxs, fys, T, txs, tys), golden["cmm_out"])
+    F, R = golden["cr_F"], golden["cr_R"]
+    for mod in range(3):
+        xl, yl, fxs, fys, txs, tys, tx3 = golden[f"cmr{mod}_args"]
+        assert np.array_equal(O.copy_mr(xl, yl, F, fxs, fys, R, txs, tys, tx3, mod),
+                              golden[f"cmr{mod}_out"]), mod
+        xl, yl, fxs, fys, fx3, txs, tys = golden[f"crm{mod}_args"]
+        assert np.array_equal(O.copy_rm(xl, yl, R, fxs, fys, fx3, mod, F, txs, tys),
+                              golden[f"crm{mod}_out"]), mod
+    a = golden["crr_args"]
+    assert np.array_equal(O.copy_rr(a[0:3], golden["crr_F"], a[3:6], R, a[6:9]),
+                          golden["crr_out"])
+
+
+def test_special_dgemm_identity_and_permutation():
+    rng = np.random.default_rng(61)
+    T = np.asfortranarray(rng.standard_normal((6, 3, 9)))
+    eye = np.eye(9)
+    out = O.special_dgemm_01(T, 1, 4, 2, 5, eye, 2, 5, 2, 5, 2.0, 0.5)
+    assert np.allclose(out[1:5, :, 2:7], 2.5 * T[1:5, :, 2:7], rtol=0, atol=1e-15)
+    assert np.array_equal(out[0], T[0]) and np.array_equal(out[:, :, :2], T[:, :, :2])
+    perm = np.eye(5)[:, [4, 0, 3, 1, 2]]
+    Bm = np.zeros((7, 7))
+    Bm[1:6, 2:7] = perm
+    out = O.special_dgemm_01(T, 0, 6, 4, 5, Bm, 1, 5, 2, 5, 1.0, 0.0)
+    assert np.array_equal(out[:, :, 4:9], T[:, :, 4:9][:, :, [4, 0, 3, 1, 2]])
