@@ -293,3 +293,19 @@ def special_dgemm_01(T, sx, lx, sz, lz, Bm, srb, lrb, scb, lcb, alpha, beta):
         blk = T[sx: sx + lx, y, sz: sz + lz]
         out[sx: sx + lx, y, sz: sz + lz] = alpha * (blk @ Bb) + beta * blk
     return out
+
+
+def rik_syrk(B: np.ndarray, chunk: int = 64) -> np.ndarray:
+    """Same K as rik(), accumulated with BLAS dsyrk (upper triangle, in place)
+    over chunks of aux slices stacked as [(m,P), i] rows, then mirrored: the
+    CPU-baseline form of the K build."""
+    from scipy.linalg.blas import dsyrk
+    nocc, nmo, naux = B.shape
+    K = np.zeros((nmo, nmo), order="F")
+    for p0 in range(0, naux, chunk):
+        blk = B[:, :, p0:p0 + chunk]
+        X = np.asfortranarray(blk.transpose(0, 2, 1).reshape(-1, nmo, order="F"))
+        K = dsyrk(1.0, X, beta=1.0, c=K, trans=1, lower=0, overwrite_c=1)
+    iu = np.triu_indices(nmo, 1)
+    K.T[iu] = K[iu]
+    return K

@@ -15,9 +15,11 @@
 // (BM/2) x (BN/2) sub-tile = (BM/32) x (BN/32) MFMA blocks.
 //
 // MFMA operand order: the B fragment is passed as src0 and the A fragment as
-// src1, so each accumulator holds C^T blocks and lane l owns rows
-// m = l%16 of columns n = 4*(l/16)+r.  Consecutive lanes then own consecutive
-// rows of the column-major output and the stores coalesce.
+// src1, so each accumulator holds C^T blocks.  For the FP64 16x16x4 MFMA,
+// accumulator register r of lane l holds output row 4*r + l/16 and column
+// l%16 (measured on gfx950; it differs from the FP32 variants' 4*(l/16) + r),
+// so here lane l owns row m = l%16 of columns n = l/16 + 4*r.  Consecutive
+// lanes own consecutive rows of the column-major output: stores coalesce.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -183,7 +185,7 @@ __global__ __launch_bounds__(256) void dgemm_mfma_kernel(GemmParams g) {
   }
 
   // epilogue: acc[i][j][r] is C(m, n) with
-  //   m = m0 + wave_m*WM + i*16 + lane%16,  n = n0 + wave_n*WN + j*16 + 4*(lane/16) + r
+  //   m = m0 + wave_m*WM + i*16 + lane%16,  n = n0 + wave_n*WN + j*16 + lane/16 + 4*r
   double* out = g.C + chunk * g.sC;
 #pragma unroll
   for (int i = 0; i < FM; ++i) {
@@ -193,7 +195,7 @@ __global__ __launch_bounds__(256) void dgemm_mfma_kernel(GemmParams g) {
     for (int j = 0; j < FN; ++j) {
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int64_t n = n0 + wave_n * WN + j * 16 + 4 * (lane >> 4) + r;
+        const int64_t n = n0 + wave_n * WN + j * 16 + (lane >> 4) + 4 * r;
         if (n >= g.N) continue;
         double* p = out + m * g.c_rs + n * g.c_cs;
         double v = acc[i][j][r];
