@@ -97,8 +97,10 @@ extern "C" int rt_ao2mo_dev(const double* C, const double* ri3ao,
                       (void**)&T));
   for (int64_t p0 = 0; p0 < naux; p0 += pc) {
     int64_t np = std::min(pc, naux - p0);
-    GemmDesc s1;  // T_P = A_P * C
-    s1.ta = false;
+    // T_P = A_P^T C (k-major operands), so that ri3mo(P,:,:) = T_P^T C
+    // equals C^T A_P C of restmatr.f90:183-188 for any (also non-symmetric) A_P
+    GemmDesc s1;
+    s1.ta = true;
     s1.tb = false;
     s1.M = nbas;
     s1.N = ns;
@@ -115,18 +117,18 @@ extern "C" int rt_ao2mo_dev(const double* C, const double* ri3ao,
     s1.sC = tslice;
     s1.batch = np;
     RTA_TRY(gemm_batched(s1, s, "ao2mo_s1"));
-    GemmDesc s2;  // ri3mo(P,:,:) = C^T T_P, aux index fastest
+    GemmDesc s2;  // ri3mo(P,:,:) = T_P^T C, aux index fastest
     s2.ta = true;
     s2.tb = false;
     s2.M = ns;
     s2.N = ns;
     s2.K = nbas;
-    s2.A = C;
+    s2.A = T;
     s2.lda = nbas;
-    s2.sA = 0;
-    s2.B = T;
+    s2.sA = tslice;
+    s2.B = C;
     s2.ldb = nbas;
-    s2.sB = tslice;
+    s2.sB = 0;
     s2.C = ri3mo + p0;
     s2.c_rs = naux;
     s2.c_cs = naux * ns;
@@ -160,32 +162,33 @@ extern "C" int rt_ao2mo_occ_dev(const double* C, int64_t nbas, int64_t nmo,
                       (void**)&X));
   for (int64_t p0 = 0; p0 < naux; p0 += pc) {
     int64_t np = std::min(pc, naux - p0);
-    GemmDesc s1;  // X_P = C_occ^T A_P
+    // X^T_P = A_P^T C_occ  (nbas x nocc, k-major for the second stage)
+    GemmDesc s1;
     s1.ta = true;
     s1.tb = false;
-    s1.M = nocc;
-    s1.N = nbas;
+    s1.M = nbas;
+    s1.N = nocc;
     s1.K = nbas;
-    s1.A = C;
+    s1.A = ri3ao + p0 * nbas * nbas;
     s1.lda = nbas;
-    s1.sA = 0;
-    s1.B = ri3ao + p0 * nbas * nbas;
+    s1.sA = nbas * nbas;
+    s1.B = C;
     s1.ldb = nbas;
-    s1.sB = nbas * nbas;
+    s1.sB = 0;
     s1.C = X;
     s1.c_rs = 1;
-    s1.c_cs = nocc;
+    s1.c_cs = nbas;
     s1.sC = xslice;
     s1.batch = np;
     RTA_TRY(gemm_batched(s1, s, "ao2mo_occ_s1"));
-    GemmDesc s2;  // B_P = X_P C
-    s2.ta = false;
+    GemmDesc s2;  // B_P = (X^T_P)^T C
+    s2.ta = true;
     s2.tb = false;
     s2.M = nocc;
     s2.N = nmo;
     s2.K = nbas;
     s2.A = X;
-    s2.lda = nocc;
+    s2.lda = nbas;
     s2.sA = xslice;
     s2.B = C;
     s2.ldb = nbas;

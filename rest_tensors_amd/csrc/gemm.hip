@@ -3,9 +3,11 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 
 #include "common.hpp"
 #include "gemm.hpp"
+#include "mfma_tn.hpp"
 #include "timer.hpp"
 
 namespace rta {
@@ -40,6 +42,107 @@ static int launch(const GemmParams& g, bool ta, bool tb, int tile,
     launch_variant<64, 64>(g, ta, tb, (unsigned)g.nwork, s);
   RTA_HIP(hipGetLastError());
   return RT_OK;
+}
+
+// ---- TN fast path (both operands k-major, direct-to-LDS pipeline) --------
+static int g_num_cu = 0;
+int device_cu_count() {
+  if (g_num_cu == 0) {
+    int dev = 0, n = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) ==
+            hipSuccess && n > 0)
+      g_num_cu = n;
+    else
+      g_num_cu = 256;
+  }
+  return g_num_cu;
+}
+
+// Engine configuration knobs (environment, read once): RTA_TN_STAGES in
+// {2,3,4} (k-chunks in flight), RTA_TN_BK in {16,32} (k-chunk depth).
+static int env_int(const char* name, int def) {
+  const char* e = getenv(name);
+  return e ? atoi(e) : def;
+}
+static int tn_stages() {
+  static int st = [] {
+    int v = env_int("RTA_TN_STAGES", 2);
+    return (v == 2 || v == 3 || v == 4) ? v : 2;
+  }();
+  return st;
+}
+static int tn_bk_pref() {
+  static int bk = [] {
+    int v = env_int("RTA_TN_BK", 32);
+    return (v == 16 || v == 32) ? v : 32;
+  }();
+  return bk;
+}
+
+bool tn_eligible(const GemmDesc& d) {
+  return d.ta && !d.tb && d.K > 0 && d.K % 16 == 0 && d.lda % 2 == 0 &&
+         d.ldb % 2 == 0 && d.sA % 2 == 0 && d.sB % 2 == 0 &&
+         ((uintptr_t)d.A & 15) == 0 && ((uintptr_t)d.B & 15) == 0;
+}
+
+template <int BM, int BN, int BK, int NS>
+static int tn_launch_cfg(const TnParams& p, hipStream_t s) {
+  const size_t lds = (size_t)NS * (BM + BN) * BK * sizeof(double);
+  static bool attr_set = false;
+  if (!attr_set) {
+    RTA_HIP(hipFuncSetAttribute((const void*)dgemm_tn_kernel<BM, BN, BK, NS>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)lds));
+    attr_set = true;
+  }
+  hipLaunchKernelGGL((dgemm_tn_kernel<BM, BN, BK, NS>), dim3((unsigned)p.nwork),
+                     dim3(256), lds, s, p);
+  RTA_HIP(hipGetLastError());
+  return RT_OK;
+}
+
+static int tn_launch(const GemmParams& g, int tile, hipStream_t s,
+                     const char* timer_name) {
+  if (g.nwork <= 0) return RT_OK;
+  RTA_CHECK(g.nwork < (int64_t(1) << 31), "GEMM grid too large");
+  TnParams p;
+  p.A = g.A;
+  p.B = g.B;
+  p.C = g.C;
+  p.M = g.M;
+  p.N = g.N;
+  p.K = g.K;
+  p.lda = g.lda;
+  p.ldb = g.ldb;
+  p.sA = g.sA;
+  p.sB = g.sB;
+  p.c_rs = g.c_rs;
+  p.c_cs = g.c_cs;
+  p.sC = g.sC;
+  p.alpha = g.alpha;
+  p.beta = g.beta;
+  p.nbatch = g.nbatch;
+  p.bchunk = g.bchunk;
+  p.nchunks = g.nchunks;
+  p.tiles_m = g.tiles_m;
+  p.tiles_n = g.tiles_n;
+  p.ntiles = g.ntiles;
+  p.nwork = g.nwork;
+  p.reduce = g.reduce;
+  p.upper = g.upper;
+  KernelTimer kt(timer_name, s);
+  const bool bk32 = tn_bk_pref() == 32 && p.K % 32 == 0;
+  if (tile == 128) {
+    if (bk32) return tn_launch_cfg<128, 128, 32, 2>(p, s);  // 2 x 64 KB LDS
+    switch (tn_stages()) {
+      case 3: return tn_launch_cfg<128, 128, 16, 3>(p, s);
+      case 4: return tn_launch_cfg<128, 128, 16, 4>(p, s);
+      default: return tn_launch_cfg<128, 128, 16, 2>(p, s);
+    }
+  }
+  if (bk32) return tn_launch_cfg<64, 64, 32, 3>(p, s);
+  return tn_launch_cfg<64, 64, 16, 4>(p, s);
 }
 
 int gemm_pick_tile(int64_t M, int64_t N, int64_t batch) {
@@ -102,6 +205,7 @@ int gemm_batched(const GemmDesc& d, hipStream_t stream, const char* timer_name,
   g.nwork = g.ntiles * g.nchunks;
   g.reduce = 0;
   g.upper = 0;
+  if (tn_eligible(d)) return tn_launch(g, tile, stream, timer_name);
   return launch(g, d.ta, d.tb, tile, stream, timer_name);
 }
 
@@ -164,6 +268,8 @@ int gemm_reduce_sym(const GemmDesc& d, int64_t bchunk, double* partial,
                        d.M, d.M, (int64_t)1, d.M, (int64_t)0, (int64_t)1, 0.0);
     RTA_HIP(hipGetLastError());
     g.nchunks = 1;
+  } else if (tn_eligible(d)) {
+    RTA_TRY(tn_launch(g, tile, stream, timer_name));
   } else {
     RTA_TRY(launch(g, d.ta, d.tb, tile, stream, timer_name));
   }
@@ -176,6 +282,32 @@ int gemm_reduce_sym(const GemmDesc& d, int64_t bchunk, double* partial,
   }
   RTA_HIP(hipGetLastError());
   return RT_OK;
+}
+
+// Batches per work item for a batch-reducing GEMM: balances ntiles*nchunks
+// work items of bchunk batches each over the CUs (one 128x128 workgroup per
+// CU), charging each extra chunk the time to stream its M x M partial.
+int64_t reduce_bchunk(int64_t M, int64_t K, int64_t nbatch, int tile) {
+  const int64_t t = (M + tile - 1) / tile;
+  const int64_t ntiles = t * (t + 1) / 2;
+  const int64_t slots = device_cu_count();
+  // microseconds: one batch of one tile on one CU (128 flop/clk at 2.4 GHz),
+  // and one M x M partial streamed at ~5 TB/s
+  const double t_unit = 2.0 * tile * tile * (double)K / (128.0 * 2.4e3);
+  const double t_part = (double)M * M * 8.0 / 5e6;
+  int64_t best = nbatch;
+  double best_cost = 1e300;
+  for (int64_t bc = 1; bc <= nbatch; ++bc) {
+    int64_t nch = (nbatch + bc - 1) / bc;
+    if (bc > 1 && (nbatch + bc - 2) / (bc - 1) == nch) continue;  // same split
+    int64_t rounds = (ntiles * nch + slots - 1) / slots;
+    double cost = (double)rounds * bc * t_unit + (double)nch * t_part;
+    if (cost < best_cost) {
+      best_cost = cost;
+      best = bc;
+    }
+  }
+  return best;
 }
 
 static int parse_op(char c, bool* t) {

@@ -30,6 +30,18 @@ int gemm_pick_tile(int64_t M, int64_t N, int64_t batch);
 int gemm_batched(const GemmDesc& d, hipStream_t stream, const char* timer_name,
                  int tile = 0);
 
+// Number of compute units of the current device (cached).
+int device_cu_count();
+
+// Whether a descriptor can run on the k-major (TN) direct-to-LDS engine:
+// ta && !tb, K % 16 == 0, even leading dims and batch strides, 16-byte
+// aligned A and B.
+bool tn_eligible(const GemmDesc& d);
+
+// Batches per workgroup for gemm_reduce_sym on an M x M output with inner
+// dimension K per batch (balances the grid over the CUs).
+int64_t reduce_bchunk(int64_t M, int64_t K, int64_t nbatch, int tile);
+
 // Batch-reducing GEMM with symmetric output:
 //   out(m,n) = alpha * sum_b op(A_b)(m,:) . op(B_b)(:,n), M == N, only the
 // upper tiles are computed, then summed over chunks of `bchunk` batches in a

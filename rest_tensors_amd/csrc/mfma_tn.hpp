@@ -1,0 +1,203 @@
+// FP64 MFMA "TN" GEMM engine for gfx950: both operands stored k-major.
+//
+//   C_slot(m,n) = sum_{b in batch range} sum_k A_b[k + m*lda] * B_b[k + n*ldb]
+//
+// Every GEMM of the RI hot path has this shape once the intermediates are
+// stored k-major:  K = sum_P B_P^T B_P,  ao2mo  X^T_P = A_P^T C_occ  and
+// B_P = (X^T_P)^T C,  reference ao2mo  T_P = A_P^T C  and  ri3mo_P = T_P^T C.
+//
+// Data path: operand tiles go global -> LDS with global_load_lds_dwordx4
+// (no register staging), NSTAGE k-chunks of BK (16 or 32) in flight.  A tile
+// row holds BK doubles = BK/2 units of 16 bytes; unit c of row r is stored at
+// position c ^ swz(r) (swz = (r>>1)&7 for 8-unit rows, r&15 for 16-unit
+// rows), which makes the ds_read_b128 fragment reads of 16 consecutive rows
+// cover all 64 LDS banks exactly once per 256 bytes: conflict-free.  A lane reads two consecutive k of its row
+// with one ds_read_b128 and feeds them to two MFMAs (the k-slot order inside
+// an MFMA is immaterial as long as both operands use the same one).
+//
+// Requirements (checked by the host launcher): K % BK == 0, lda, ldb, sA, sB
+// even, A and B 16-byte aligned.  Rows beyond M/N are clamped to the last
+// valid row on load and masked on store.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "mfma_f64.hpp"
+
+namespace rta {
+
+typedef __attribute__((address_space(3))) void lds_void_t;
+typedef __attribute__((address_space(1))) void glb_void_t;
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+
+struct TnParams {
+  const double* A;
+  const double* B;
+  double* C;
+  int64_t M, N, K;
+  int64_t lda, ldb;
+  int64_t sA, sB;
+  int64_t c_rs, c_cs, sC;
+  double alpha, beta;
+  int64_t nbatch, bchunk, nchunks;
+  int tiles_m, tiles_n;
+  int64_t ntiles, nwork;
+  int reduce, upper;
+};
+
+template <int N>
+__device__ __forceinline__ void wait_vmcnt() {
+  if constexpr (N == 0) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+  else if constexpr (N == 4) asm volatile("s_waitcnt vmcnt(4)" ::: "memory");
+  else if constexpr (N == 8) asm volatile("s_waitcnt vmcnt(8)" ::: "memory");
+  else if constexpr (N == 12) asm volatile("s_waitcnt vmcnt(12)" ::: "memory");
+  else if constexpr (N == 16) asm volatile("s_waitcnt vmcnt(16)" ::: "memory");
+  else if constexpr (N == 24) asm volatile("s_waitcnt vmcnt(24)" ::: "memory");
+  else if constexpr (N == 32) asm volatile("s_waitcnt vmcnt(32)" ::: "memory");
+  else static_assert(N == 0, "unsupported vmcnt");
+}
+
+template <int BK>
+__device__ __forceinline__ int tn_swz(int r) {
+  return BK == 16 ? ((r >> 1) & 7) : (r & 15);
+}
+
+// Issues the direct-to-LDS loads of one BK-deep k-chunk of an R-row operand
+// tile: R*BK/2 units of 16 B, R*BK/512 instructions per wave (4 waves).
+template <int R, int BK>
+__device__ __forceinline__ void tn_issue(const double* __restrict__ base,
+                                         int64_t ld, int64_t r0, int64_t rmax,
+                                         int64_t koff, double* lds_tile) {
+  constexpr int UPR = BK / 2;                 // units per row
+  constexpr int NI = R * UPR / 256;           // instructions per wave
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < NI; ++q) {
+    const int inst = wave * NI + q;
+    const int u = inst * 64 + lane;
+    const int row = u / UPR, pos = u % UPR;
+    const int c = pos ^ tn_swz<BK>(row);
+    int64_t gr = r0 + row;
+    gr = gr < rmax ? gr : rmax - 1;
+    const double* src = base + gr * ld + koff + 2 * c;
+    __builtin_amdgcn_global_load_lds((const glb_void_t*)src,
+                                     (lds_void_t*)(lds_tile + inst * 128), 16,
+                                     0, 0);
+  }
+}
+
+template <int BM, int BN, int BK, int NSTAGE>
+__global__ __launch_bounds__(256) void dgemm_tn_kernel(TnParams g) {
+  static_assert(BK == 16 || BK == 32, "BK");
+  constexpr int WM = BM / 2, WN = BN / 2;
+  constexpr int FM = WM / 16, FN = WN / 16;
+  constexpr int STAGE = (BM + BN) * BK;  // doubles per stage
+  extern __shared__ double smem[];
+
+  const int64_t w = xcd_remap(blockIdx.x, gridDim.x);
+  if (w >= g.nwork) return;
+  const int64_t chunk = w / g.ntiles;
+  const int64_t tid = w % g.ntiles;
+  int tm, tn;
+  if (g.upper) {
+    upper_tile(tid, tm, tn);
+  } else {
+    tm = (int)(tid % g.tiles_m);
+    tn = (int)(tid / g.tiles_m);
+  }
+  const int64_t m0 = (int64_t)tm * BM, n0 = (int64_t)tn * BN;
+  const int64_t b0 = chunk * g.bchunk;
+  const int64_t b1 = (b0 + g.bchunk < g.nbatch) ? b0 + g.bchunk : g.nbatch;
+  const int64_t nk = g.K / BK;
+  const int64_t niter = (b1 - b0) * nk;
+
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int wave_m = wave & 1, wave_n = wave >> 1;
+
+  f64x4 acc[FM][FN];
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) acc[i][j] = f64x4{0.0, 0.0, 0.0, 0.0};
+
+  auto issue = [&](int64_t it) {
+    const int64_t b = b0 + it / nk;
+    const int64_t k0 = (it % nk) * BK;
+    double* st = smem + (it % NSTAGE) * STAGE;
+    tn_issue<BM, BK>(g.A + b * g.sA, g.lda, m0, g.M, k0, st);
+    tn_issue<BN, BK>(g.B + b * g.sB, g.ldb, n0, g.N, k0, st + BM * BK);
+  };
+
+  // prologue: NSTAGE-1 chunks in flight
+#pragma unroll
+  for (int s = 0; s < NSTAGE - 1; ++s)
+    if (s < niter) issue(s);
+
+  constexpr int PER_STAGE = (BM + BN) * BK / 512;  // load instructions per wave
+  for (int64_t it = 0; it < niter; ++it) {
+    // this wave's loads of chunk `it` have landed once at most the loads of
+    // the younger chunks are outstanding
+    const int64_t younger = (niter - 1 - it) < (NSTAGE - 2) ? (niter - 1 - it)
+                                                            : (NSTAGE - 2);
+    if (younger >= 2) wait_vmcnt<2 * PER_STAGE>();
+    else if (younger == 1) wait_vmcnt<PER_STAGE>();
+    else wait_vmcnt<0>();
+    // every wave's chunk `it` is in LDS and chunk it-1 is consumed (its LDS
+    // reads completed before its MFMAs).  A bare s_barrier: __syncthreads()
+    // would also drain vmcnt and serialize the load pipeline.
+    __builtin_amdgcn_s_barrier();
+    if (it + NSTAGE - 1 < niter) issue(it + NSTAGE - 1);
+
+    const double* As = smem + (it % NSTAGE) * STAGE;
+    const double* Bs = As + BM * BK;
+#pragma unroll
+    for (int kp = 0; kp < BK / 8; ++kp) {
+      const int c = kp * 4 + (lane >> 4);
+      f64x2 a[FM], b[FN];
+#pragma unroll
+      for (int i = 0; i < FM; ++i) {
+        const int r = wave_m * WM + i * 16 + (lane & 15);
+        a[i] = *(const f64x2*)(As + r * BK + 2 * (c ^ tn_swz<BK>(r)));
+      }
+#pragma unroll
+      for (int j = 0; j < FN; ++j) {
+        const int r = wave_n * WN + j * 16 + (lane & 15);
+        b[j] = *(const f64x2*)(Bs + r * BK + 2 * (c ^ tn_swz<BK>(r)));
+      }
+#pragma unroll
+      for (int s = 0; s < 2; ++s)
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+          for (int j = 0; j < FN; ++j)
+            acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[j][s], a[i][s],
+                                                             acc[i][j], 0, 0, 0);
+    }
+  }
+
+  // epilogue: lane l owns row m = .. + l%16 of columns n = .. + l/16 + 4*r
+  double* out = g.C + chunk * g.sC;
+#pragma unroll
+  for (int i = 0; i < FM; ++i) {
+    const int64_t m = m0 + wave_m * WM + i * 16 + (lane & 15);
+    if (m >= g.M) continue;
+#pragma unroll
+    for (int j = 0; j < FN; ++j) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t n = n0 + wave_n * WN + j * 16 + (lane >> 4) + 4 * r;
+        if (n >= g.N) continue;
+        double* p = out + m * g.c_rs + n * g.c_cs;
+        const double v = acc[i][j][r];
+        if (g.reduce) *p = v;
+        else if (g.beta == 0.0) *p = g.alpha * v;
+        else *p = g.alpha * v + g.beta * (*p);
+      }
+    }
+  }
+}
+
+}  // namespace rta

@@ -114,20 +114,10 @@ __global__ __launch_bounds__(256) void occ_sumsq_kernel(
   if (threadIdx.x == 0) partial[P] = (red[0] + red[1]) + (red[2] + red[3]);
 }
 
-// Chunk size over P for the K reduction: enough chunks that the upper-tile x
-// chunk grid covers the 256 CUs several times.
-static int64_t rik_bchunk(int64_t nmo, int64_t naux, int tile) {
-  int64_t t = (nmo + tile - 1) / tile;
-  int64_t ntiles = t * (t + 1) / 2;
-  int64_t want = std::max<int64_t>(1, (4 * 256 + ntiles - 1) / ntiles);
-  want = std::min<int64_t>(want, naux);
-  return (naux + want - 1) / want;
-}
-
 int rik_fp64(const double* B, int64_t nocc, int64_t nmo, int64_t naux,
              double* K, hipStream_t s) {
   const int tile = 128;
-  int64_t bchunk = rik_bchunk(nmo, naux, tile);
+  int64_t bchunk = reduce_bchunk(nmo, nocc, naux, tile);
   int64_t nchunks = (naux + bchunk - 1) / bchunk;
   double* part = nullptr;
   RTA_TRY(scratch_get(SCRATCH_GEMM_PARTIAL,

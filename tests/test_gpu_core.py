@@ -42,7 +42,8 @@ def dp(a):
 @pytest.mark.parametrize("opa", "NT")
 @pytest.mark.parametrize("opb", "NT")
 @pytest.mark.parametrize("mnk", [(1, 1, 1), (17, 5, 33), (64, 64, 16), (130, 257, 67),
-                                 (300, 129, 1000)])
+                                 (300, 129, 1000), (130, 257, 96), (5, 3, 32),
+                                 (256, 384, 512)])
 def test_dgemm_ops_shapes(D, opa, opb, mnk):
     m, n, k = mnk
     rng = np.random.default_rng(m * 7 + n * 3 + k)
@@ -72,6 +73,29 @@ def test_dgemm_batched_big_tiles(D):
         Aq = A[q].reshape(-1).reshape((m, k), order="F")
         Bq = B[q].reshape(-1).reshape((k, n), order="F")
         assert relf(out[q].reshape(-1).reshape((m, n), order="F"), Aq @ Bq) < 1e-13
+
+
+@pytest.mark.parametrize("mnkb", [(256, 256, 96, 128), (1024, 128, 64, 24), (77, 130, 32, 9)])
+def test_dgemm_batched_kmajor(D, mnkb):
+    # op(A) = A^T, op(B) = B with k-major storage: the direct-to-LDS engine
+    from rest_tensors_amd._lib import lib, check
+    m, n, k, b = mnkb
+    rng = np.random.default_rng(m + n + k + b)
+    A = rng.standard_normal((b, m, k))   # per batch k x m column-major (k fastest)
+    B = rng.standard_normal((b, n, k))   # per batch k x n column-major
+    dA, dB = D.DeviceArray.from_host(A.reshape(-1)), D.DeviceArray.from_host(B.reshape(-1))
+    Cm = rng.standard_normal((b, n, m))
+    dC = D.DeviceArray.from_host(Cm.reshape(-1))
+    check(lib().rt_dgemm_strided_batched_dev(b"T", b"N", m, n, k, 1.5, dA.ptr, k, m * k,
+                                             dB.ptr, k, k * n, 0.25, dC.ptr, m, m * n, b,
+                                             None))
+    out = dC.to_host().reshape(b, n, m)
+    for q in (0, b // 2, b - 1):
+        Aq = A[q].reshape(-1).reshape((k, m), order="F")
+        Bq = B[q].reshape(-1).reshape((k, n), order="F")
+        Cq = Cm[q].reshape(-1).reshape((m, n), order="F")
+        ref = 1.5 * Aq.T @ Bq + 0.25 * Cq
+        assert relf(out[q].reshape(-1).reshape((m, n), order="F"), ref) < 1e-13
 
 
 def test_dgemm_rejects_bad_shapes(D):
