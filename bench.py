@@ -45,7 +45,23 @@ def main():
                     help="aux slices of the CPU-baseline K sample (0 = skip)")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the ao2mo / RI-J side measurements")
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="only run one plain device copy of 512 MiB (copy3d kernel: reads "
+                         "512 MiB, writes 512 MiB) to calibrate FETCH_SIZE / WRITE_SIZE")
     args = ap.parse_args()
+
+    if args.traffic_probe:
+        from rest_tensors_amd import device as D
+        from rest_tensors_amd._lib import lib, check
+        n = 64 * 1024 * 1024
+        src, dst = D.DeviceArray(n), D.DeviceArray(n)
+        D.gen_normal(src, n, seed=1)
+        D.synchronize()
+        check(lib().rt_copy_mm_dev(n, 1, src.ptr, n, 1, 0, 0, dst.ptr, n, 1, 0, 0, None))
+        D.synchronize()
+        print(json.dumps({"traffic_probe": "copy3d_kernel", "read_KiB": n * 8 // 1024,
+                          "write_KiB": n * 8 // 1024}))
+        return
 
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; "

@@ -181,6 +181,105 @@ int rt_copy_rr_dev(int64_t x_len, int64_t y_len, int64_t z_len,
                    int64_t t_x, int64_t t_y, int64_t t_z, int64_t t_xs,
                    int64_t t_ys, int64_t t_zs, void* stream);
 
+/* ---- Davidson eigensolver (A12) ------------------------------------------ */
+/* Replaces davidson_solve (src/davidson.rs:48-266).  The reference's closure
+ * a_x(&Vec<f64>) -> Vec<f64> becomes a block operator on device vectors:
+ * Y[:, c] = A X[:, c] for c < k (column-major, leading dim n).  Return RT_OK
+ * on success. */
+typedef int (*rt_matvec_fn)(void* ctx, const double* X, double* Y, int64_t n,
+                            int64_t k, void* stream);
+
+#define RT_PRECOND_DIAG_REF 0  /* r / (hdiag - e[0]) for every root (davidson.rs:224) */
+#define RT_PRECOND_DIAG_ROOT 1 /* r / (hdiag - e[k]) */
+#define RT_PRECOND_MLP 2       /* learned MLP correction (rt_mlp_*) */
+
+/* Optional recorder of (residual, Ritz vector, e_k, e_0, root, cycle) per
+ * root per cycle, the training data of the learned preconditioner. */
+typedef struct rt_davidson_record {
+  double* r;     /* device, n * cap */
+  double* x;     /* device, n * cap */
+  double* e_k;   /* host, cap */
+  double* e_0;   /* host, cap */
+  int* root;     /* host, cap */
+  int* iter;     /* host, cap */
+  int64_t cap;
+  int64_t count; /* in/out: samples recorded so far */
+} rt_davidson_record;
+
+/* DavidsonParams (davidson.rs:28-46) plus the preconditioner choice. */
+typedef struct rt_davidson_params {
+  double tol;      /* |de| < tol and |r| < sqrt(tol) (davidson.rs:189) */
+  int maxcyc;
+  int maxspace;    /* grown by 3*(nroots-1) as in davidson.rs:74 */
+  double lindep;
+  int nroots;
+  int precond;     /* RT_PRECOND_* */
+  const void* mlp; /* network for RT_PRECOND_MLP */
+  rt_davidson_record* record; /* may be NULL */
+  int verbose;
+} rt_davidson_params;
+
+typedef struct rt_davidson_stats {
+  int iterations;
+  int matvecs;
+  int restarts;
+  int nroots_found;
+  int mlp_fallbacks;
+  double seconds;      /* wall time of the solve */
+  double max_residual; /* max |r| of the last cycle */
+} rt_davidson_stats;
+
+/* x0: device initial guess (one vector, as davidson.rs:73); hdiag: device
+ * diagonal of A; conv, e: host [nroots]; x_out: device n x nroots (may be
+ * NULL).  Returns RT_ERR_NOTCONV if maxcyc is reached (outputs still set). */
+int rt_davidson_solve_dev(rt_matvec_fn a_x, void* ctx, const double* x0,
+                          const double* hdiag, int64_t n,
+                          const rt_davidson_params* params, int* conv,
+                          double* e, double* x_out, rt_davidson_stats* stats,
+                          void* stream);
+
+/* ---- RI linear-response operator of the Davidson workload (A15) --------- */
+/* A = diag(d) + 2 scale^2 B B^T, B: device n x naux block with leading dim
+ * ldb (e.g. the occ-vir block of ri3mo(occ,all,aux): B = ri3mo + nocc^2,
+ * ldb = nocc*nmo).  The handle does not own B or d. */
+int rt_casida_op_create(const double* B, int64_t ldb, int64_t n, int64_t naux,
+                        double scale, const double* d, void** op);
+int rt_casida_op_destroy(void* op);
+int rt_casida_matvec(void* op, const double* X, double* Y, int64_t n,
+                     int64_t k, void* stream); /* an rt_matvec_fn */
+int rt_casida_hdiag(void* op, double* hdiag, void* stream);
+
+/* ---- learned preconditioner: 2-layer FP32 MLP (hidden 64) ------------------ */
+int rt_mlp_num_params(void);
+int rt_mlp_create(uint64_t seed, void** net);
+int rt_mlp_destroy(void* net);
+int rt_mlp_get_params(void* net, float* out);
+int rt_mlp_set_params(void* net, const float* in);
+/* T[:, s] = network correction for residual R[:, s], Ritz vector X[:, s] and
+ * shifts ek[s], e0[s] (host arrays), s < ns. */
+int rt_mlp_apply_dev(void* net, const double* R, const double* X,
+                     const double* hdiag, const double* ek, const double* e0,
+                     int64_t n, int ns, double* T, void* stream);
+/* loss (host scalar) and its gradient (host [rt_mlp_num_params()]) over ns
+ * samples with device targets tau (n x ns) and host weights wts[ns]. */
+int rt_mlp_loss_grad_dev(void* net, const double* R, const double* X,
+                         const double* hdiag, const double* ek,
+                         const double* e0, const double* tau, const double* wts,
+                         int64_t n, int ns, double* loss, double* grad,
+                         void* stream);
+/* Adam over `steps` minibatches of `batch` samples (seeded draw);
+ * loss_hist: host [steps] or NULL. */
+int rt_mlp_train_dev(void* net, const double* R, const double* X,
+                     const double* hdiag, const double* ek, const double* e0,
+                     const double* tau, const double* wts, int64_t n, int ns,
+                     int steps, int batch, double lr, uint64_t seed,
+                     double* loss_hist, void* stream);
+/* targets tau[:, s] = normalize(x* - (x*.x) x) with x* = Xfinal[:, root[s]]
+ * (host root[]); wts[s] = 0 where |x*.x| < min_overlap (root mismatch). */
+int rt_mlp_targets_dev(const double* X, const int* root, int ns,
+                       const double* Xfinal, int64_t n, double min_overlap,
+                       double* tau, double* wts, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

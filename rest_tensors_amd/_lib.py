@@ -68,6 +68,23 @@ RT_SIGNATURES = {
                              i64, i64, vp]),
     "rt_copy_rr_dev": (i32, [i64, i64, i64, vp, i64, i64, i64, i64, i64, i64, vp, i64, i64,
                              i64, i64, i64, i64, vp]),
+    # Davidson / RI operator / MLP
+    "rt_davidson_solve_dev": (i32, [vp, vp, vp, vp, i64, vp, pi32, pf64, vp, vp, vp]),
+    "rt_casida_op_create": (i32, [vp, i64, i64, i64, f64, vp, C.POINTER(vp)]),
+    "rt_casida_op_destroy": (i32, [vp]),
+    "rt_casida_matvec": (i32, [vp, vp, vp, i64, i64, vp]),
+    "rt_casida_hdiag": (i32, [vp, vp, vp]),
+    "rt_mlp_num_params": (i32, []),
+    "rt_mlp_create": (i32, [u64, C.POINTER(vp)]),
+    "rt_mlp_destroy": (i32, [vp]),
+    "rt_mlp_get_params": (i32, [vp, C.POINTER(C.c_float)]),
+    "rt_mlp_set_params": (i32, [vp, C.POINTER(C.c_float)]),
+    "rt_mlp_apply_dev": (i32, [vp, vp, vp, vp, pf64, pf64, i64, i32, vp, vp]),
+    "rt_mlp_loss_grad_dev": (i32, [vp, vp, vp, vp, pf64, pf64, vp, pf64, i64, i32, pf64,
+                                   pf64, vp]),
+    "rt_mlp_train_dev": (i32, [vp, vp, vp, vp, pf64, pf64, vp, pf64, i64, i32, i32, i32,
+                               f64, u64, pf64, vp]),
+    "rt_mlp_targets_dev": (i32, [vp, pi32, i32, vp, i64, f64, vp, pf64, vp]),
 }
 
 # the seven drop-in symbols (include/restmatr.h): every argument by reference

@@ -30,6 +30,12 @@ enum ScratchSlot {
   SCRATCH_DROPIN_C,
   SCRATCH_DROPIN_D,
   SCRATCH_JK,
+  SCRATCH_DV_Z,
+  SCRATCH_DV_YPART,
+  SCRATCH_LOWP_A,
+  SCRATCH_LOWP_B,
+  SCRATCH_MLP_A,
+  SCRATCH_MLP_B,
   SCRATCH_NSLOTS
 };
 int scratch_get(int slot, size_t bytes, void** ptr);

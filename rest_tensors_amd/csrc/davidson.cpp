@@ -1,0 +1,390 @@
+// Davidson eigensolver (A12), restated from davidson_solve (src/davidson.rs:48-266)
+// with every vector resident in HBM.  The host keeps only the small
+// subspace matrix heff and the control flow; dots, norms, Gram-Schmidt and
+// Ritz-vector assembly run as device kernels (dvkernels.hip) on one stream.
+//
+// Kept from the reference: one initial guess vector, maxspace grown by
+// 3*(nroots-1) (:74), fresh restarts when space + nroots > maxspace (:260),
+// at most 40 new vectors (:115-117), the convergence test |de| < tol and
+// |r| < sqrt(tol) (:189), the diagonal preconditioner with the shift e[0]
+// for every root (:224), sequential projection against the subspace
+// (:235-243) and the lindep drops (:268-325).  The "large |r|" restore
+// (:210-217) compares against a value that is never updated (:259 shadows
+// it); the same constant 1e9 is used here.  Where the reference keeps using
+// a pre-QR vector count after _qr dropped vectors (:112-136, an out-of-range
+// index there), the post-QR count is used.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "common.hpp"
+#include "dvkernels.hpp"
+#include "mlp.hpp"
+
+namespace rta {
+
+// Cyclic Jacobi eigensolver for a small symmetric matrix (column-major,
+// n x n).  Eigenvalues ascending in w, eigenvectors in the columns of V.
+static void jacobi_eigh(const std::vector<double>& A_in, int n,
+                        std::vector<double>& w, std::vector<double>& V) {
+  std::vector<double> A(A_in);
+  V.assign((size_t)n * n, 0.0);
+  for (int i = 0; i < n; ++i) V[i + (size_t)n * i] = 1.0;
+  auto a = [&](int i, int j) -> double& { return A[i + (size_t)n * j]; };
+  for (int sweep = 0; sweep < 100; ++sweep) {
+    double off = 0.0, tot = 0.0;
+    for (int j = 0; j < n; ++j)
+      for (int i = 0; i < n; ++i) {
+        tot += a(i, j) * a(i, j);
+        if (i != j) off += a(i, j) * a(i, j);
+      }
+    if (off <= 1e-30 * tot || off == 0.0) break;
+    for (int p = 0; p < n - 1; ++p)
+      for (int q = p + 1; q < n; ++q) {
+        const double apq = a(p, q);
+        if (apq == 0.0) continue;
+        const double theta = (a(q, q) - a(p, p)) / (2.0 * apq);
+        const double t = (theta >= 0 ? 1.0 : -1.0) /
+                         (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
+        const double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
+        for (int k = 0; k < n; ++k) {  // columns p, q
+          const double akp = a(k, p), akq = a(k, q);
+          a(k, p) = c * akp - s * akq;
+          a(k, q) = s * akp + c * akq;
+        }
+        for (int k = 0; k < n; ++k) {  // rows p, q
+          const double apk = a(p, k), aqk = a(q, k);
+          a(p, k) = c * apk - s * aqk;
+          a(q, k) = s * apk + c * aqk;
+        }
+        for (int k = 0; k < n; ++k) {
+          const double vkp = V[k + (size_t)n * p], vkq = V[k + (size_t)n * q];
+          V[k + (size_t)n * p] = c * vkp - s * vkq;
+          V[k + (size_t)n * q] = s * vkp + c * vkq;
+        }
+      }
+  }
+  std::vector<int> idx(n);
+  for (int i = 0; i < n; ++i) idx[i] = i;
+  std::stable_sort(idx.begin(), idx.end(),
+                   [&](int x, int y) { return a(x, x) < a(y, y); });
+  std::vector<double> V2((size_t)n * n);
+  w.resize(n);
+  for (int k = 0; k < n; ++k) {
+    w[k] = a(idx[k], idx[k]);
+    for (int i = 0; i < n; ++i)
+      V2[i + (size_t)n * k] = V[i + (size_t)n * idx[k]];
+  }
+  V.swap(V2);
+}
+
+struct DevBuf {
+  double* p = nullptr;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  int alloc(size_t count) {
+    hipError_t e = hipMalloc(&p, sizeof(double) * std::max<size_t>(count, 1));
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      p = nullptr;
+      return fail(RT_ERR_NOMEM, "davidson: device allocation of %zu doubles failed",
+                  count);
+    }
+    return RT_OK;
+  }
+};
+
+static double now_s() {
+  using namespace std::chrono;
+  return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace rta
+
+using namespace rta;
+
+extern "C" int rt_davidson_solve_dev(rt_matvec_fn a_x, void* ctx,
+                                     const double* x0_dev,
+                                     const double* hdiag_dev, int64_t n,
+                                     const rt_davidson_params* prm, int* conv_out,
+                                     double* e_out, double* x_out,
+                                     rt_davidson_stats* stats, void* stream) {
+  RTA_CHECK(a_x && prm && x0_dev && hdiag_dev && conv_out && e_out,
+            "NULL argument");
+  const int nroots = prm->nroots;
+  RTA_CHECK(nroots >= 1 && nroots <= 64, "nroots must be in [1, 64]");
+  RTA_CHECK(n >= 1, "empty problem");
+  RTA_CHECK(n - nroots >= 3, "Too much nroots required (n - nroots < 3)");
+  RTA_CHECK(prm->precond != RT_PRECOND_MLP || prm->mlp,
+            "MLP preconditioner requested without a network");
+  hipStream_t s = (hipStream_t)stream;
+  const double t_start = now_s();
+  const double tol = prm->tol, lindep = prm->lindep;
+  const int maxspace = prm->maxspace + (nroots - 1) * 3;
+  const int cap = maxspace + 2 * nroots + 48;
+  const int tmax = std::max(nroots, 40);
+
+  DevBuf XS, AX, X0, AX0, R, T, work, scr, small;
+  RTA_TRY(XS.alloc((size_t)n * cap));
+  RTA_TRY(AX.alloc((size_t)n * cap));
+  RTA_TRY(X0.alloc((size_t)n * cap));
+  RTA_TRY(AX0.alloc((size_t)n * cap));
+  RTA_TRY(R.alloc((size_t)n * tmax));
+  RTA_TRY(T.alloc((size_t)n * tmax));
+  RTA_TRY(work.alloc((size_t)n));
+  const int64_t scr_n =
+      std::max<int64_t>(dv_gram_scratch(n, cap, tmax), dv_gram_scratch(n, 1, 1) * tmax);
+  RTA_TRY(scr.alloc((size_t)scr_n));
+  // small device area: e (cap), shift (cap), scale (cap), sq (cap), coef (cap*cap),
+  // gram out (cap*tmax), int count
+  const size_t o_e = 0, o_shift = cap, o_scale = 2 * (size_t)cap,
+               o_sq = 3 * (size_t)cap, o_coef = 4 * (size_t)cap,
+               o_gram = o_coef + (size_t)cap * cap,
+               o_cnt = o_gram + (size_t)cap * tmax;
+  RTA_TRY(small.alloc(o_cnt + 2));
+  double* d_e = small.p + o_e;
+  double* d_shift = small.p + o_shift;
+  double* d_scale = small.p + o_scale;
+  double* d_sq = small.p + o_sq;
+  double* d_coef = small.p + o_coef;
+  double* d_gram = small.p + o_gram;
+  int* d_cnt = (int*)(small.p + o_cnt);
+
+  RTA_HIP(hipMemcpyAsync(X0.p, x0_dev, sizeof(double) * n,
+                         hipMemcpyDeviceToDevice, s));
+  int nx0 = 1;           // vectors in X0
+  int nt = 0;            // new vectors in T (non-fresh)
+  int space = 0;
+  bool fresh = true;
+  std::vector<double> heff, w, V, e, elast;
+  std::vector<int> conv(nroots, 0);
+  std::vector<double> dx_norm, de, h_sq(cap), h_gram((size_t)cap * tmax),
+      h_coef((size_t)cap * cap), h_shift(cap), h_scale(cap);
+  int nroots_current = 0;
+  rt_davidson_stats st;
+  memset(&st, 0, sizeof st);
+  int status = RT_ERR_NOTCONV;
+
+  for (int icyc = 0; icyc < prm->maxcyc; ++icyc) {
+    int xtlen = 0;
+    if (fresh) {
+      space = 0;
+      RTA_TRY(dv_mgs(X0.p, nx0, n, lindep, XS.p, d_cnt, work.p, s));
+      RTA_HIP(hipMemcpyAsync(&xtlen, d_cnt, sizeof(int), hipMemcpyDeviceToHost, s));
+      RTA_HIP(hipStreamSynchronize(s));
+      if (xtlen == 0)
+        return fail(RT_ERR_NUMERIC, "No more linear independent basis found");
+      ++st.restarts;
+    } else if (nt > 1) {
+      RTA_TRY(dv_mgs(T.p, nt, n, lindep, XS.p + (size_t)space * n, d_cnt, work.p, s));
+      RTA_HIP(hipMemcpyAsync(&xtlen, d_cnt, sizeof(int), hipMemcpyDeviceToHost, s));
+      RTA_HIP(hipStreamSynchronize(s));
+      xtlen = std::min(xtlen, 40);
+    } else {
+      RTA_HIP(hipMemcpyAsync(XS.p + (size_t)space * n, T.p, sizeof(double) * n,
+                             hipMemcpyDeviceToDevice, s));
+      xtlen = nt;
+    }
+    if (space + xtlen > cap)
+      return fail(RT_ERR_NUMERIC, "subspace overflow (%d + %d > %d)", space, xtlen, cap);
+    // A x for the new vectors, written straight into the AX block
+    RTA_TRY(a_x(ctx, XS.p + (size_t)space * n, AX.p + (size_t)space * n, n,
+                xtlen, stream));
+    st.matvecs += xtlen;
+    const int row0 = space;
+    space += xtlen;
+    // heff: H[i,j] = H[j,i] = ax_i . x_j for i <= j, j new (davidson.rs:356-387)
+    RTA_TRY(dv_gram(AX.p, n, space, XS.p + (size_t)row0 * n, n, xtlen, n, d_gram,
+                    space, scr.p, s));
+    RTA_HIP(hipMemcpyAsync(h_gram.data(), d_gram, sizeof(double) * space * xtlen,
+                           hipMemcpyDeviceToHost, s));
+    RTA_HIP(hipStreamSynchronize(s));
+    {
+      std::vector<double> hn((size_t)space * space, 0.0);
+      if (!fresh) {
+        for (int j = 0; j < row0; ++j)
+          for (int i = 0; i < row0; ++i)
+            hn[i + (size_t)space * j] = heff[i + (size_t)row0 * j];
+      }
+      for (int jj = 0; jj < xtlen; ++jj) {
+        const int j = row0 + jj;
+        for (int i = 0; i <= j; ++i) {
+          const double v = h_gram[i + (size_t)space * jj];
+          hn[i + (size_t)space * j] = v;
+          hn[j + (size_t)space * i] = v;
+        }
+      }
+      heff.swap(hn);
+    }
+    elast = e;
+    jacobi_eigh(heff, space, w, V);
+    const int nkeep = std::min(space, nroots);
+    e.assign(w.begin(), w.begin() + nkeep);
+    nroots_current = nkeep;
+    nx0 = nkeep;
+    for (int c = 0; c < nx0; ++c)
+      for (int i = 0; i < space; ++i)
+        h_coef[i + (size_t)space * c] = V[i + (size_t)space * c];
+    RTA_HIP(hipMemcpyAsync(d_coef, h_coef.data(), sizeof(double) * space * nx0,
+                           hipMemcpyHostToDevice, s));
+    RTA_HIP(hipMemcpyAsync(d_e, e.data(), sizeof(double) * nkeep,
+                           hipMemcpyHostToDevice, s));
+    RTA_TRY(dv_combine(XS.p, n, space, d_coef, space, nx0, n, X0.p, n, s));
+    RTA_TRY(dv_combine(AX.p, n, space, d_coef, space, nx0, n, AX0.p, n, s));
+    RTA_TRY(dv_residuals(AX0.p, X0.p, d_e, nroots_current, n, R.p, d_sq, scr.p, s));
+    RTA_HIP(hipMemcpyAsync(h_sq.data(), d_sq, sizeof(double) * nroots_current,
+                           hipMemcpyDeviceToHost, s));
+    RTA_HIP(hipStreamSynchronize(s));
+    dx_norm.assign(nroots_current, 0.0);
+    de.assign(nroots_current, 0.0);
+    bool all_conv = true;
+    double max_dx = 0.0;
+    for (int k = 0; k < nroots_current; ++k) {
+      de[k] = (k < (int)elast.size()) ? e[k] - elast[k] : e[k];
+      dx_norm[k] = std::sqrt(h_sq[k]);
+      conv[k] = (std::fabs(de[k]) < tol && dx_norm[k] < std::sqrt(tol)) ? 1 : 0;
+      max_dx = std::max(max_dx, dx_norm[k]);
+    }
+    for (int k = 0; k < nroots; ++k) all_conv = all_conv && conv[k];
+    if (prm->record) {
+      rt_davidson_record* rec = prm->record;
+      for (int k = 0; k < nroots_current && rec->count < rec->cap; ++k) {
+        const int64_t q = rec->count++;
+        RTA_HIP(hipMemcpyAsync(rec->r + q * n, R.p + (size_t)k * n,
+                               sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+        RTA_HIP(hipMemcpyAsync(rec->x + q * n, X0.p + (size_t)k * n,
+                               sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+        rec->e_k[q] = e[k];
+        rec->e_0[q] = e[0];
+        rec->root[q] = k;
+        rec->iter[q] = icyc;
+      }
+    }
+    st.iterations = icyc + 1;
+    st.max_residual = max_dx;
+    if (prm->verbose)
+      fprintf(stderr, "davidson cyc %d space %d |r|max %.3e e0 %.12f\n", icyc, space,
+              max_dx, e[0]);
+    if (all_conv) {
+      status = RT_OK;
+      break;
+    }
+    if (max_dx > 1.0 && max_dx / 1e9 > 3.0 && space > nroots + 2) {
+      fresh = true;  // davidson.rs:210-217 (restart from the current Ritz vectors)
+      continue;
+    }
+    // corrections for roots with |r|^2 > lindep (davidson.rs:220-233)
+    std::vector<int> keep;
+    for (int k = 0; k < nroots_current; ++k)
+      if (dx_norm[k] * dx_norm[k] > lindep) keep.push_back(k);
+    nt = (int)keep.size();
+    // compact the kept residuals / Ritz vectors to the front
+    for (int j = 0; j < nt; ++j)
+      if (keep[j] != j) {
+        RTA_HIP(hipMemcpyAsync(R.p + (size_t)j * n, R.p + (size_t)keep[j] * n,
+                               sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+      }
+    if (nt > 0) {
+      if (prm->precond == RT_PRECOND_MLP) {
+        RTA_TRY(mlp_precondition((const MlpNet*)prm->mlp, R.p, X0.p, keep.data(), nt,
+                                 hdiag_dev, e.data(), n, T.p, scr.p, s,
+                                 &st.mlp_fallbacks));
+      } else {
+        for (int j = 0; j < nt; ++j)
+          h_shift[j] = (prm->precond == RT_PRECOND_DIAG_ROOT) ? e[keep[j]] : e[0];
+        RTA_HIP(hipMemcpyAsync(d_shift, h_shift.data(), sizeof(double) * nt,
+                               hipMemcpyHostToDevice, s));
+        RTA_TRY(dv_precond_diag(R.p, hdiag_dev, d_shift, nt, n, T.p, s));
+      }
+      // normalize, project out the subspace, renormalize with lindep drops
+      RTA_TRY(dv_sqnorms(T.p, nt, n, d_sq, scr.p, s));
+      RTA_HIP(hipMemcpyAsync(h_sq.data(), d_sq, sizeof(double) * nt,
+                             hipMemcpyDeviceToHost, s));
+      RTA_HIP(hipStreamSynchronize(s));
+      for (int j = 0; j < nt; ++j) h_scale[j] = 1.0 / std::sqrt(h_sq[j]);
+      RTA_HIP(hipMemcpyAsync(d_scale, h_scale.data(), sizeof(double) * nt,
+                             hipMemcpyHostToDevice, s));
+      RTA_TRY(dv_scale_cols(T.p, d_scale, nt, n, s));
+      RTA_TRY(dv_project_out(XS.p, n, space, T.p, nt, n, s));
+      RTA_TRY(dv_sqnorms(T.p, nt, n, d_sq, scr.p, s));
+      RTA_HIP(hipMemcpyAsync(h_sq.data(), d_sq, sizeof(double) * nt,
+                             hipMemcpyDeviceToHost, s));
+      RTA_HIP(hipStreamSynchronize(s));
+      int m = 0;
+      for (int j = 0; j < nt; ++j) {
+        if (h_sq[j] > lindep) {
+          if (m != j)
+            RTA_HIP(hipMemcpyAsync(T.p + (size_t)m * n, T.p + (size_t)j * n,
+                                   sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+          h_scale[m] = 1.0 / std::sqrt(h_sq[j]);
+          ++m;
+        }
+      }
+      nt = m;
+      if (nt > 0) {
+        RTA_HIP(hipMemcpyAsync(d_scale, h_scale.data(), sizeof(double) * nt,
+                               hipMemcpyHostToDevice, s));
+        RTA_TRY(dv_scale_cols(T.p, d_scale, nt, n, s));
+      }
+    }
+    if (nt == 0) {
+      // linear dependency in the trial space (davidson.rs:250-257)
+      bool ok = true;
+      for (int k = 0; k < nroots; ++k) {
+        if (k < (int)dx_norm.size())
+          conv[k] = conv[k] || dx_norm[k] < std::sqrt(tol);
+        ok = ok && conv[k];
+      }
+      status = ok ? RT_OK : RT_ERR_NOTCONV;
+      break;
+    }
+    fresh = space + nroots > maxspace;
+  }
+  RTA_HIP(hipStreamSynchronize(s));
+  for (int k = 0; k < nroots; ++k) conv_out[k] = conv[k];
+  for (int k = 0; k < nroots; ++k)
+    e_out[k] = (k < (int)e.size()) ? e[k] : std::nan("");
+  if (x_out && nroots_current > 0)
+    RTA_HIP(hipMemcpyAsync(x_out, X0.p, sizeof(double) * n * nroots_current,
+                           hipMemcpyDeviceToDevice, s));
+  RTA_HIP(hipStreamSynchronize(s));
+  st.nroots_found = nroots_current;
+  st.seconds = now_s() - t_start;
+  if (stats) *stats = st;
+  if (status != RT_OK)
+    fail(status, "davidson did not converge in %d cycles (max |r| %.3e)",
+         prm->maxcyc, st.max_residual);
+  return status;
+}
+
+// ---- the RI linear-response operator behind the C ABI -----------------------------
+extern "C" int rt_casida_op_create(const double* B, int64_t ldb, int64_t n,
+                                   int64_t naux, double scale, const double* d,
+                                   void** op) {
+  RTA_CHECK(op && B && d, "NULL argument");
+  RTA_CHECK(n > 0 && naux >= 0 && ldb >= n, "bad operator shape");
+  CasidaOp* o = new CasidaOp{B, ldb, n, naux, scale, d};
+  *op = o;
+  return RT_OK;
+}
+
+extern "C" int rt_casida_op_destroy(void* op) {
+  delete (CasidaOp*)op;
+  return RT_OK;
+}
+
+extern "C" int rt_casida_matvec(void* op, const double* X, double* Y, int64_t n,
+                                int64_t k, void* stream) {
+  RTA_CHECK(op, "NULL operator");
+  return casida_apply((const CasidaOp*)op, X, Y, n, k, (hipStream_t)stream);
+}
+
+extern "C" int rt_casida_hdiag(void* op, double* hdiag, void* stream) {
+  RTA_CHECK(op && hdiag, "NULL argument");
+  return casida_hdiag((const CasidaOp*)op, hdiag, (hipStream_t)stream);
+}

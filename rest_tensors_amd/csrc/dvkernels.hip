@@ -1,0 +1,458 @@
+// Device kernels of the Davidson solver (A12/A15): vector algebra on
+// device-resident subspaces, the diagonal preconditioners, and the skinny
+// products of the RI linear-response operator A = diag(d) + 2 s^2 B B^T.
+//
+// Vectors are columns of length n in column-major blocks (leading dim n).
+// All reductions run in a fixed order (deterministic results).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "common.hpp"
+#include "dvkernels.hpp"
+#include "timer.hpp"
+
+namespace rta {
+
+__device__ __forceinline__ double dv_wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Block-wide sum of one value per thread (blockDim.x multiple of 64, <= 1024).
+__device__ double dv_block_sum(double v, double* red) {
+  v = dv_wave_sum(v);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int nw = blockDim.x >> 6;
+  __syncthreads();
+  if (lane == 0) red[wave] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0) {
+    for (int i = 0; i < nw; ++i) s += red[i];
+    red[32] = s;
+  }
+  __syncthreads();
+  s = red[32];
+  return s;
+}
+
+// ---- Gram blocks: G[i + ldg*j] = U[:, i] . W[:, j] ---------------------------
+// Split over x into `nsplit` slices; partial[s][i + nu*j]; then a fixed-order sum.
+__global__ __launch_bounds__(256) void gram_partial_kernel(
+    const double* __restrict__ U, int64_t ldu, int nu,
+    const double* __restrict__ W, int64_t ldw, int nw, int64_t n, int64_t slice,
+    double* __restrict__ partial) {
+  __shared__ double red[40];
+  const int64_t x0 = (int64_t)blockIdx.x * slice;
+  const int64_t x1 = (x0 + slice < n) ? x0 + slice : n;
+  for (int p = 0; p < nu * nw; ++p) {
+    const int i = p % nu, j = p / nu;
+    const double* u = U + (int64_t)i * ldu;
+    const double* w = W + (int64_t)j * ldw;
+    double acc = 0.0;
+    for (int64_t x = x0 + threadIdx.x; x < x1; x += 256) acc += u[x] * w[x];
+    acc = dv_block_sum(acc, red);
+    if (threadIdx.x == 0) partial[(int64_t)blockIdx.x * nu * nw + p] = acc;
+  }
+}
+
+__global__ void gram_reduce_kernel(const double* __restrict__ partial,
+                                   int nsplit, int nu, int nw,
+                                   double* __restrict__ G, int64_t ldg) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= nu * nw) return;
+  double s = 0.0;
+  for (int k = 0; k < nsplit; ++k) s += partial[(int64_t)k * nu * nw + p];
+  G[(p % nu) + ldg * (p / nu)] = s;
+}
+
+static int gram_nsplit(int64_t n) {
+  return (int)std::max<int64_t>(1, std::min<int64_t>(256, (n + 2047) / 2048));
+}
+
+int dv_gram(const double* U, int64_t ldu, int nu, const double* W, int64_t ldw,
+            int nw, int64_t n, double* G, int64_t ldg, double* scratch,
+            hipStream_t s) {
+  if (nu <= 0 || nw <= 0) return RT_OK;
+  const int ns = gram_nsplit(n);
+  const int64_t slice = (n + ns - 1) / ns;
+  KernelTimer kt("dv_gram", s);
+  hipLaunchKernelGGL(gram_partial_kernel, dim3(ns), dim3(256), 0, s, U, ldu,
+                     nu, W, ldw, nw, n, slice, scratch);
+  hipLaunchKernelGGL(gram_reduce_kernel, dim3((nu * nw + 255) / 256),
+                     dim3(256), 0, s, scratch, ns, nu, nw, G, ldg);
+  RTA_HIP(hipGetLastError());
+  return RT_OK;
+}
+
+int64_t dv_gram_scratch(int64_t n, int nu, int nw) {
+  return (int64_t)gram_nsplit(n) * nu * nw;
+}
+
+// ---- Y[:, c] = sum_s V[:, s] coef[s + ldc*c]  (tall times small) ---------------
+__global__ __launch_bounds__(256) void combine_kernel(
+    const double* __restrict__ V, int64_t ldv, int ns,
+    const double* __restrict__ coef, int ldc, int nc, int64_t n,
+    double* __restrict__ Y, int64_t ldy) {
+  const int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int c = blockIdx.y;
+  if (x >= n || c >= nc) return;
+  double acc = 0.0;
+  for (int s = 0; s < ns; ++s) acc += V[x + s * ldv] * coef[s + ldc * c];
+  Y[x + c * ldy] = acc;
+}
+
+int dv_combine(const double* V, int64_t ldv, int ns, const double* coef,
+               int ldc, int nc, int64_t n, double* Y, int64_t ldy,
+               hipStream_t s) {
+  if (nc <= 0 || n <= 0) return RT_OK;
+  hipLaunchKernelGGL(combine_kernel, dim3((unsigned)((n + 255) / 256), nc),
+                     dim3(256), 0, s, V, ldv, ns, coef, ldc, nc, n, Y, ldy);
+  RTA_HIP(hipGetLastError());
+  return RT_OK;
+}
+
+// ---- residuals R[:,k] = AX0[:,k] - e_k X0[:,k] and their norms ------------------
+__global__ __launch_bounds__(256) void residual_kernel(
+    const double* __restrict__ AX0, const double* __restrict__ X0,
+    const double* __restrict__ e, int64_t n, double* __restrict__ R,
+    double* __restrict__ sq_partial, int64_t slice) {
+  __shared__ double red[40];
+  const int k = blockIdx.y;
+  const int64_t x0 = (int64_t)blockIdx.x * slice;
+  const int64_t x1 = (x0 + slice < n) ? x0 + slice : n;
+  const double ek = e[k];
+  double acc = 0.0;
+  for (int64_t x = x0 + threadIdx.x; x < x1; x += 256) {
+    double r = AX0[x + k * n] - ek * X0[x + k * n];
+    R[x + k * n] = r;
+    acc += r * r;
+  }
+  acc = dv_block_sum(acc, red);
+  if (threadIdx.x == 0) sq_partial[k * gridDim.x + blockIdx.x] = acc;
+}
+
+// sum of squares of columns: out[k] = sum_s partial[k*ns + s]
+__global__ void colsum_kernel(const double* __restrict__ partial, int ns,
+                              int nk, double* __restrict__ out) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nk) return;
+  double s = 0.0;
+  for (int i = 0; i < ns; ++i) s += partial[k * ns + i];
+  out[k] = s;
+}
+
+int dv_residuals(const double* AX0, const double* X0, const double* e_dev,
+                 int nk, int64_t n, double* R, double* sq_out, double* scratch,
+                 hipStream_t s) {
+  if (nk <= 0) return RT_OK;
+  const int ns = gram_nsplit(n);
+  const int64_t slice = (n + ns - 1) / ns;
+  hipLaunchKernelGGL(residual_kernel, dim3(ns, nk), dim3(256), 0, s, AX0, X0,
+                     e_dev, n, R, scratch, slice);
+  hipLaunchKernelGGL(colsum_kernel, dim3(1), dim3(64), 0, s, scratch, ns, nk,
+                     sq_out);
+  RTA_HIP(hipGetLastError());
+  return RT_OK;
+}
+
+// squared norms of nk columns (ld n)
+__global__ __launch_bounds__(256) void sqnorm_kernel(const double* __restrict__ X,
+                                                     int64_t n, int64_t slice,
+                                                     double* __restrict__ part) {
+  __shared__ double red[40];
+  const int k = blockIdx.y;
+  const int64_t x0 = (int64_t)blockIdx.x * slice;
+  const int64_t x1 = (x0 + slice < n) ? x0 + slice : n;
+  double acc = 0.0;
+  for (int64_t x = x0 + threadIdx.x; x < x1; x += 256) {
+    double v = X[x + k * n];
+    acc += v * v;
+  }
+  acc = dv_block_sum(acc, red);
+  if (threadIdx.x == 0) part[k * gridDim.x + blockIdx.x] = acc;
+}
+
+int dv_sqnorms(const double* X, int nk, int64_t n, double* sq_out,
+               double* scratch, hipStream_t s) {
+  if (nk <= 0) return RT_OK;
+  const int ns = gram_nsplit(n);
+  const int64_t slice = (n + ns - 1) / ns;
+  hipLaunchKernelGGL(sqnorm_kernel, dim3(ns, nk), dim3(256), 0, s, X, n, slice,
+                     scratch);
+  hipLaunchKernelGGL(colsum_kernel, dim3(1), dim3(64), 0, s, scratch, ns, nk,
+                     sq_out);
+  RTA_HIP(hipGetLastError());
+  return RT_OK;
+}
+
+// ---- preconditioners --------------------------------------------------------------
+// T[:,k] = R[:,k] / (hdiag - shift[k]) with |h| < 1e-8 -> 1e-8 (davidson.rs:63-70)
+__global__ __launch_bounds__(256) void precond_diag_kernel(
+    const double* __restrict__ R, const double* __restrict__ hdiag,
+    const double* __restrict__ shift, int64_t n, double* __restrict__ T) {
+  const int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int k = blockIdx.y;
+  if (x >= n) return;
+  double h = hdiag[x] - shift[k];
+  if (fabs(h) < 1e-8) h = 1e-8;
+  T[x + k * n] = R[x + k * n] / h;
+}
+
+int dv_precond_diag(const double* R, const double* hdiag, const double* shift,
+                    int nk, int64_t n, double* T, hipStream_t s) {
+  if (nk <= 0) return RT_OK;
+  hipLaunchKernelGGL(precond_diag_kernel, dim3((unsigned)((n + 255) / 256), nk),
+                     dim3(256), 0, s, R, hdiag, shift, n, T);
+  RTA_HIP(hipGetLastError());
+  return RT_OK;
+}
+
+// X[:, k] *= scale[k]
+__global__ void scale_cols_kernel(double* __restrict__ X,
+                                  const double* __restrict__ scale, int64_t n) {
+  const int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int k = blockIdx.y;
+  if (x < n) X[x + k * n] *= scale[k];
+}
+
+int dv_scale_cols(double* X, const double* scale, int nk, int64_t n,
+                  hipStream_t s) {
+  if (nk <= 0) return RT_OK;
+  hipLaunchKernelGGL(scale_cols_kernel, dim3((unsigned)((n + 255) / 256), nk),
+                     dim3(256), 0, s, X, scale, n);
+  RTA_HIP(hipGetLastError());
+  return RT_OK;
+}
+
+// ---- sequential Gram-Schmidt -----------------------------------------------------
+// One workgroup per column k of T: for i in 0..ns-1 (in order):
+//   T[:,k] -= Q[:,i] (Q[:,i] . T[:,k])       (davidson.rs:235-243)
+__global__ __launch_bounds__(1024) void project_out_kernel(
+    const double* __restrict__ Q, int64_t ldq, int ns, double* __restrict__ T,
+    int64_t n) {
+  __shared__ double red[40];
+  double* t = T + (int64_t)blockIdx.x * n;
+  for (int i = 0; i < ns; ++i) {
+    const double* q = Q + (int64_t)i * ldq;
+    double acc = 0.0;
+    for (int64_t x = threadIdx.x; x < n; x += blockDim.x) acc += q[x] * t[x];
+    const double d = dv_block_sum(acc, red);
+    for (int64_t x = threadIdx.x; x < n; x += blockDim.x) t[x] -= q[x] * d;
+    __syncthreads();
+  }
+}
+
+int dv_project_out(const double* Q, int64_t ldq, int ns, double* T, int nt,
+                   int64_t n, hipStream_t s) {
+  if (ns <= 0 || nt <= 0) return RT_OK;
+  KernelTimer kt("dv_project", s);
+  hipLaunchKernelGGL(project_out_kernel, dim3(nt), dim3(1024), 0, s, Q, ldq, ns,
+                     T, n);
+  RTA_HIP(hipGetLastError());
+  return RT_OK;
+}
+
+// Modified Gram-Schmidt with dropping (_qr, davidson.rs:287-325), one
+// workgroup: columns of X (ld n) are processed in order; accepted ones are
+// normalized and written to Q (ld n) consecutively; count -> *nout.
+__global__ __launch_bounds__(1024) void mgs_kernel(const double* __restrict__ X,
+                                                   int nx, int64_t n,
+                                                   double lindep,
+                                                   double* __restrict__ Q,
+                                                   int* __restrict__ nout,
+                                                   double* __restrict__ work) {
+  __shared__ double red[40];
+  int nv = 0;
+  for (int i = 0; i < nx; ++i) {
+    const double* xi = X + (int64_t)i * n;
+    for (int64_t x = threadIdx.x; x < n; x += blockDim.x) work[x] = xi[x];
+    __syncthreads();
+    for (int j = 0; j < nv; ++j) {
+      const double* q = Q + (int64_t)j * n;
+      double acc = 0.0;
+      for (int64_t x = threadIdx.x; x < n; x += blockDim.x) acc += q[x] * work[x];
+      const double prod = dv_block_sum(acc, red);
+      for (int64_t x = threadIdx.x; x < n; x += blockDim.x) work[x] -= q[x] * prod;
+      __syncthreads();
+    }
+    double acc = 0.0;
+    for (int64_t x = threadIdx.x; x < n; x += blockDim.x) acc += work[x] * work[x];
+    const double nn = dv_block_sum(acc, red);
+    const double norm = sqrt(nn);
+    if (norm * norm > lindep) {
+      double* q = Q + (int64_t)nv * n;
+      for (int64_t x = threadIdx.x; x < n; x += blockDim.x) q[x] = work[x] / norm;
+      ++nv;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *nout = nv;
+}
+
+int dv_mgs(const double* X, int nx, int64_t n, double lindep, double* Q,
+           int* nout_dev, double* work, hipStream_t s) {
+  KernelTimer kt("dv_mgs", s);
+  hipLaunchKernelGGL(mgs_kernel, dim3(1), dim3(1024), 0, s, X, nx, n, lindep, Q,
+                     nout_dev, work);
+  RTA_HIP(hipGetLastError());
+  return RT_OK;
+}
+
+// ---- RI linear-response operator --------------------------------------------------
+// Z[P + naux*c] = sum_x B[x + ldb*P] X[x + n*c]   (8 aux columns per block)
+template <int KMAX>
+__global__ __launch_bounds__(256) void skinny_bt_x_kernel(
+    const double* __restrict__ B, int64_t ldb, int64_t naux,
+    const double* __restrict__ X, int64_t n, int k, double* __restrict__ Z) {
+  constexpr int PG = 8;
+  __shared__ double red[PG * KMAX][5];
+  const int64_t P0 = (int64_t)blockIdx.x * PG;
+  double acc[PG][KMAX];
+#pragma unroll
+  for (int p = 0; p < PG; ++p)
+#pragma unroll
+    for (int c = 0; c < KMAX; ++c) acc[p][c] = 0.0;
+  for (int64_t x = threadIdx.x; x < n; x += 256) {
+    double xv[KMAX];
+#pragma unroll
+    for (int c = 0; c < KMAX; ++c) xv[c] = (c < k) ? X[x + n * c] : 0.0;
+#pragma unroll
+    for (int p = 0; p < PG; ++p) {
+      const double b = (P0 + p < naux) ? B[x + ldb * (P0 + p)] : 0.0;
+#pragma unroll
+      for (int c = 0; c < KMAX; ++c) acc[p][c] += b * xv[c];
+    }
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int p = 0; p < PG; ++p)
+#pragma unroll
+    for (int c = 0; c < KMAX; ++c) {
+      double v = dv_wave_sum(acc[p][c]);
+      if (lane == 0) red[p * KMAX + c][wave] = v;
+    }
+  __syncthreads();
+  for (int q = threadIdx.x; q < PG * KMAX; q += 256) {
+    const int p = q / KMAX, c = q % KMAX;
+    if (P0 + p < naux && c < k)
+      Z[(P0 + p) + naux * c] = (red[q][0] + red[q][1]) + (red[q][2] + red[q][3]);
+  }
+}
+
+// Ypart[s][x + n*c] = sum_{P in slice s} B[x + ldb*P] Z[P + naux*c]
+template <int KMAX>
+__global__ __launch_bounds__(256) void skinny_b_z_kernel(
+    const double* __restrict__ B, int64_t ldb, int64_t naux,
+    const double* __restrict__ Z, int64_t n, int k, int64_t pslice,
+    double* __restrict__ Ypart) {
+  const int64_t x = blockIdx.x * (int64_t)256 + threadIdx.x;
+  const int64_t s = blockIdx.y;
+  const int64_t P0 = s * pslice;
+  const int64_t P1 = (P0 + pslice < naux) ? P0 + pslice : naux;
+  double acc[KMAX];
+#pragma unroll
+  for (int c = 0; c < KMAX; ++c) acc[c] = 0.0;
+  if (x < n) {
+    const double* b = B + x;
+    int64_t P = P0;
+    for (; P + 3 < P1; P += 4) {
+      const double b0 = b[ldb * P], b1 = b[ldb * (P + 1)], b2 = b[ldb * (P + 2)],
+                   b3 = b[ldb * (P + 3)];
+#pragma unroll
+      for (int c = 0; c < KMAX; ++c) {
+        if (c < k) {
+          const double* z = Z + naux * c + P;
+          acc[c] += b0 * z[0];
+          acc[c] += b1 * z[1];
+          acc[c] += b2 * z[2];
+          acc[c] += b3 * z[3];
+        }
+      }
+    }
+    for (; P < P1; ++P) {
+      const double bv = b[ldb * P];
+#pragma unroll
+      for (int c = 0; c < KMAX; ++c)
+        if (c < k) acc[c] += bv * Z[naux * c + P];
+    }
+#pragma unroll
+    for (int c = 0; c < KMAX; ++c)
+      if (c < k) Ypart[s * n * k + x + n * c] = acc[c];
+  }
+}
+
+// Y[x + n*c] = d[x] X[x + n*c] + alpha * sum_s Ypart[s][x + n*c]
+__global__ __launch_bounds__(256) void skinny_finish_kernel(
+    const double* __restrict__ Ypart, int nslice, const double* __restrict__ d,
+    const double* __restrict__ X, double alpha, int64_t n, int k,
+    double* __restrict__ Y) {
+  const int64_t e = blockIdx.x * (int64_t)256 + threadIdx.x;
+  if (e >= n * k) return;
+  const int64_t x = e % n;
+  double s = 0.0;
+  for (int i = 0; i < nslice; ++i) s += Ypart[(int64_t)i * n * k + e];
+  Y[e] = d[x] * X[e] + alpha * s;
+}
+
+// hdiag[x] = d[x] + alpha * sum_P B[x + ldb*P]^2
+__global__ __launch_bounds__(256) void casida_hdiag_kernel(
+    const double* __restrict__ B, int64_t ldb, int64_t naux,
+    const double* __restrict__ d, double alpha, int64_t n,
+    double* __restrict__ hdiag) {
+  const int64_t x = blockIdx.x * (int64_t)256 + threadIdx.x;
+  if (x >= n) return;
+  double s = 0.0;
+  for (int64_t P = 0; P < naux; ++P) {
+    const double b = B[x + ldb * P];
+    s += b * b;
+  }
+  hdiag[x] = d[x] + alpha * s;
+}
+
+int casida_apply(const CasidaOp* op, const double* X, double* Y, int64_t n,
+                 int64_t k, hipStream_t s) {
+  RTA_CHECK(n == op->n, "vector length %lld != operator dimension %lld",
+            (long long)n, (long long)op->n);
+  if (k <= 0) return RT_OK;
+  const double alpha = 2.0 * op->scale * op->scale;
+  for (int64_t c0 = 0; c0 < k; c0 += 8) {
+    const int kc = (int)std::min<int64_t>(8, k - c0);
+    const double* Xc = X + c0 * n;
+    double* Yc = Y + c0 * n;
+    double *Z, *Ypart;
+    RTA_TRY(scratch_get(SCRATCH_DV_Z, sizeof(double) * op->naux * 8, (void**)&Z));
+    const int nslice = 8;
+    const int64_t pslice = (op->naux + nslice - 1) / nslice;
+    RTA_TRY(scratch_get(SCRATCH_DV_YPART, sizeof(double) * nslice * n * 8,
+                        (void**)&Ypart));
+    {
+      KernelTimer kt("casida_btx", s);
+      hipLaunchKernelGGL(skinny_bt_x_kernel<8>, dim3((unsigned)((op->naux + 7) / 8)),
+                         dim3(256), 0, s, op->B, op->ldb, op->naux, Xc, n, kc, Z);
+    }
+    {
+      KernelTimer kt("casida_bz", s);
+      hipLaunchKernelGGL(skinny_b_z_kernel<8>,
+                         dim3((unsigned)((n + 255) / 256), nslice), dim3(256), 0,
+                         s, op->B, op->ldb, op->naux, Z, n, kc, pslice, Ypart);
+    }
+    hipLaunchKernelGGL(skinny_finish_kernel, dim3((unsigned)((n * kc + 255) / 256)),
+                       dim3(256), 0, s, Ypart, nslice, op->d, Xc, alpha, n, kc,
+                       Yc);
+    RTA_HIP(hipGetLastError());
+  }
+  return RT_OK;
+}
+
+int casida_hdiag(const CasidaOp* op, double* hdiag, hipStream_t s) {
+  hipLaunchKernelGGL(casida_hdiag_kernel, dim3((unsigned)((op->n + 255) / 256)),
+                     dim3(256), 0, s, op->B, op->ldb, op->naux, op->d,
+                     2.0 * op->scale * op->scale, op->n, hdiag);
+  RTA_HIP(hipGetLastError());
+  return RT_OK;
+}
+
+}  // namespace rta

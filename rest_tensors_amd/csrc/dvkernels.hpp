@@ -1,0 +1,48 @@
+// Device building blocks of the Davidson solver and the RI linear-response
+// operator (see dvkernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace rta {
+
+// A = diag(d) + 2 scale^2 B B^T, B an n x naux column block (leading dim ldb).
+struct CasidaOp {
+  const double* B;
+  int64_t ldb;
+  int64_t n;
+  int64_t naux;
+  double scale;
+  const double* d;  // device, length n
+};
+
+int casida_apply(const CasidaOp* op, const double* X, double* Y, int64_t n,
+                 int64_t k, hipStream_t s);
+int casida_hdiag(const CasidaOp* op, double* hdiag, hipStream_t s);
+
+// G[i + ldg*j] = U[:,i] . W[:,j], i < nu, j < nw; scratch >= dv_gram_scratch
+int dv_gram(const double* U, int64_t ldu, int nu, const double* W, int64_t ldw,
+            int nw, int64_t n, double* G, int64_t ldg, double* scratch,
+            hipStream_t s);
+int64_t dv_gram_scratch(int64_t n, int nu, int nw);
+// Y[:,c] = sum_s V[:,s] coef[s + ldc*c]
+int dv_combine(const double* V, int64_t ldv, int ns, const double* coef,
+               int ldc, int nc, int64_t n, double* Y, int64_t ldy,
+               hipStream_t s);
+// R[:,k] = AX0[:,k] - e[k] X0[:,k]; sq_out[k] = |R[:,k]|^2 (device)
+int dv_residuals(const double* AX0, const double* X0, const double* e_dev,
+                 int nk, int64_t n, double* R, double* sq_out, double* scratch,
+                 hipStream_t s);
+int dv_sqnorms(const double* X, int nk, int64_t n, double* sq_out,
+               double* scratch, hipStream_t s);
+int dv_precond_diag(const double* R, const double* hdiag, const double* shift,
+                    int nk, int64_t n, double* T, hipStream_t s);
+int dv_scale_cols(double* X, const double* scale, int nk, int64_t n,
+                  hipStream_t s);
+int dv_project_out(const double* Q, int64_t ldq, int ns, double* T, int nt,
+                   int64_t n, hipStream_t s);
+int dv_mgs(const double* X, int nx, int64_t n, double lindep, double* Q,
+           int* nout_dev, double* work, hipStream_t s);
+
+}  // namespace rta

@@ -1,0 +1,184 @@
+"""GPU Davidson, RI linear-response operator and learned preconditioner
+against the oracles (oracle/davidson_oracle.py, oracle/mlp_oracle.py)."""
+import numpy as np
+import pytest
+
+from oracle import davidson_oracle as DO
+from oracle import mlp_oracle as MO
+from oracle import ri_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def D():
+    from rest_tensors_amd import device
+    assert device.device_count() >= 1, "no GPU visible"
+    return device
+
+
+@pytest.fixture(scope="module")
+def problem(D):
+    """The Davidson workload's construction at the CPU-reference size:
+    nbas=128, naux=384, nocc=16 -> n = 16*112 = 1792."""
+    from rest_tensors_amd import davidson as DV
+    nbas, naux, nocc = 128, 384, 16
+    dA = D.DeviceArray(nbas * nbas * naux)
+    D.gen_ri3ao(dA, nbas, naux, seed=11)
+    Cm = O.gen_orthonormal(nbas, seed=12)
+    dC = D.DeviceArray.from_host(Cm)
+    dB = D.DeviceArray(nocc * nbas * naux)
+    D.ao2mo_occ(dC, nbas, nbas, nocc, dA, naux, dB)
+    Bf = dB.to_host((nocc, nbas, naux))
+    eps = DO.casida_eps(nbas, nocc, seed=13)
+    dvec, Bov = DO.casida_problem(Bf, eps, nocc)
+    n = len(dvec)
+    dd = D.DeviceArray.from_host(dvec)
+    op = DV.CasidaOp(dB, nocc * nbas, n, naux, 0.05, dd, offset=nocc * nocc)
+    A = np.diag(dvec) + 2 * Bov @ Bov.T
+    hd = D.DeviceArray(n)
+    op.hdiag(hd)
+    return dict(op=op, A=A, n=n, hd=hd, dvec=dvec, Bov=Bov, keep=(dA, dC, dB, dd))
+
+
+def test_casida_matvec_and_diagonal(D, problem):
+    A, n, op = problem["A"], problem["n"], problem["op"]
+    assert np.abs(problem["hd"].to_host() - np.diag(A)).max() < 1e-13
+    rng = np.random.default_rng(1)
+    for k in (1, 5, 8, 11):
+        X = rng.standard_normal((n, k))
+        dX, dY = D.DeviceArray.from_host(X), D.DeviceArray(n * k)
+        op.apply(dX, dY, k)
+        Y = dY.to_host((n, k))
+        ref = A @ X
+        assert np.linalg.norm(Y - ref) / np.linalg.norm(ref) < 1e-13
+
+
+def x0_guess(D, hd_host):
+    x0 = np.zeros(len(hd_host))
+    x0[np.argmin(hd_host)] = 1.0
+    return D.DeviceArray.from_host(x0), x0
+
+
+@pytest.mark.parametrize("precond", ["diag_ref", "diag_root"])
+def test_davidson_matches_oracle(D, problem, precond):
+    from rest_tensors_amd import davidson as DV
+    A, n, op, hd = problem["A"], problem["n"], problem["op"], problem["hd"]
+    hdh = hd.to_host()
+    dx0, x0 = x0_guess(D, hdh)
+    nroots = 8
+    res = DV.solve(op.matvec_fn, op.handle, dx0, hd, n, nroots, tol=1e-12, maxcyc=300,
+                   maxspace=12, precond=precond)
+    pc = None if precond == "diag_ref" else \
+        (lambda k, rk, e, x: DO.precond_diag(rk, hdh, e[k]))
+    ref = DO.davidson_solve(lambda xs: [A @ x for x in xs], x0, hdh,
+                            DO.DavidsonParams(tol=1e-12, maxcyc=300, nroots=nroots,
+                                              maxspace=12), precond=pc)
+    assert all(res.conv) and all(ref.conv)
+    assert abs(res.stats["iterations"] - ref.iterations) <= 1
+    w = np.linalg.eigvalsh(A)[:nroots]
+    assert np.abs(res.e - w).max() < 1e-10
+    X = res.x.to_host((n, nroots))
+    resid = np.linalg.norm(A @ X - X * res.e[None, :], axis=0)
+    assert resid.max() < 1e-6     # the residual judges convergence
+    print("iterations gpu/oracle:", res.stats["iterations"], ref.iterations,
+          "matvecs:", res.stats["matvecs"], ref.matvecs, "seconds:", res.stats["seconds"])
+
+
+def test_davidson_recorder_counts(D, problem):
+    from rest_tensors_amd import davidson as DV
+    n, op, hd = problem["n"], problem["op"], problem["hd"]
+    dx0, _ = x0_guess(D, hd.to_host())
+    rec = DV.Recorder(n, 2000)
+    res = DV.solve(op.matvec_fn, op.handle, dx0, hd, n, 4, tol=1e-10, maxcyc=200,
+                   maxspace=12, recorder=rec)
+    assert all(res.conv)
+    assert rec.count >= res.stats["iterations"]
+    assert set(rec.root[:rec.count].tolist()) <= set(range(4))
+    # a recorded residual is A x - e x of its recorded Ritz vector
+    q = rec.count - 1
+    r = rec.r.to_host()[q * n:(q + 1) * n]
+    x = rec.x.to_host()[q * n:(q + 1) * n]
+    A = problem["A"]
+    assert np.linalg.norm(A @ x - rec.e_k[q] * x - r) < 1e-12
+
+
+def mlp_batch(D, problem, ns=6, seed=3):
+    n, hd = problem["n"], problem["hd"].to_host()
+    rng = np.random.default_rng(seed)
+    R = 1e-3 * rng.standard_normal((n, ns))
+    X = rng.standard_normal((n, ns))
+    X /= np.linalg.norm(X, axis=0)
+    T = rng.standard_normal((n, ns))
+    T /= np.linalg.norm(T, axis=0)
+    ek = 0.4 + 0.01 * np.arange(ns)
+    e0 = np.full(ns, 0.4)
+    return R, X, T, ek, e0, hd
+
+
+def test_mlp_forward_matches_oracle(D, problem):
+    from rest_tensors_amd import davidson as DV
+    n = problem["n"]
+    R, X, _, ek, e0, hd = mlp_batch(D, problem)
+    ns = R.shape[1]
+    net = DV.Mlp(seed=1)
+    p0 = net.get_params()
+    assert np.array_equal(p0, MO.init_params(1))
+    rng = np.random.default_rng(2)
+    for p in (p0, p0 + 0.2 * rng.standard_normal(p0.size).astype(np.float32)):
+        net.set_params(p)
+        dT = D.DeviceArray(n * ns)
+        net.apply(D.DeviceArray.from_host(R), D.DeviceArray.from_host(X), problem["hd"],
+                  ek, e0, n, ns, dT)
+        T = dT.to_host((n, ns))
+        for s in range(ns):
+            ref, _ = MO.forward(p, R[:, s], X[:, s], hd, ek[s], e0[s])
+            # FP32 network on the device, FP64 oracle
+            assert np.linalg.norm(T[:, s] - ref) / np.linalg.norm(ref) < 1e-5
+
+
+def test_mlp_gradient_matches_oracle(D, problem):
+    from rest_tensors_amd import davidson as DV
+    n = problem["n"]
+    R, X, Tau, ek, e0, hd = mlp_batch(D, problem, ns=5, seed=8)
+    wts = np.array([1.0, 1.0, 0.0, 0.5, 1.0])
+    net = DV.Mlp(seed=4)
+    p = net.get_params() + 0.1 * np.random.default_rng(5).standard_normal(708).astype(np.float32)
+    net.set_params(p)
+    L, g = net.loss_grad(D.DeviceArray.from_host(R), D.DeviceArray.from_host(X),
+                         problem["hd"], ek, e0, D.DeviceArray.from_host(Tau), wts, n, 5)
+    samples = [(R[:, s], X[:, s], hd, ek[s], e0[s]) for s in range(5)]
+    Lr, gr = MO.loss_grad(p, samples, [Tau[:, s] for s in range(5)], wts)
+    assert abs(L - Lr) < 1e-5 * max(1.0, abs(Lr))
+    assert np.linalg.norm(g - gr) / np.linalg.norm(gr) < 1e-3
+
+
+def test_mlp_training_and_preconditioned_solve(D, problem):
+    """Train on plain-Davidson samples of this problem, then solve with the
+    network: converged to the same eigenvalues, residual still the judge."""
+    from rest_tensors_amd import davidson as DV
+    A, n, op, hd = problem["A"], problem["n"], problem["op"], problem["hd"]
+    dx0, _ = x0_guess(D, hd.to_host())
+    rec = DV.Recorder(n, 600)
+    plain = DV.solve(op.matvec_fn, op.handle, dx0, hd, n, 8, tol=1e-12, maxcyc=300,
+                     maxspace=12, recorder=rec)
+    ns = rec.count
+    tau = D.DeviceArray(n * ns)
+    wts = DV.mlp_targets(rec.x, rec.root[:ns], ns, plain.x, n, tau)
+    assert wts.sum() > 10
+    net = DV.Mlp(seed=1)
+    L0, _ = net.loss_grad(rec.r, rec.x, hd, rec.e_k[:ns], rec.e_0[:ns], tau, wts, n, ns)
+    hist = net.train(rec.r, rec.x, hd, rec.e_k[:ns], rec.e_0[:ns], tau, wts, n, ns,
+                     steps=150, batch=32, lr=3e-3)
+    L1, _ = net.loss_grad(rec.r, rec.x, hd, rec.e_k[:ns], rec.e_0[:ns], tau, wts, n, ns)
+    print("training loss", L0, "->", L1, "(first/last minibatch", hist[0], hist[-1], ")")
+    assert L1 < L0
+    res = DV.solve(op.matvec_fn, op.handle, dx0, hd, n, 8, tol=1e-12, maxcyc=300,
+                   maxspace=12, precond="mlp", mlp=net)
+    assert all(res.conv)
+    w = np.linalg.eigvalsh(A)[:8]
+    assert np.abs(res.e - w).max() < 1e-10
+    X = res.x.to_host((n, 8))
+    assert np.linalg.norm(A @ X - X * res.e[None, :], axis=0).max() < 1e-6
+    print("plain iterations", plain.stats["iterations"], "mlp iterations",
+          res.stats["iterations"])
