@@ -123,6 +123,42 @@ def main():
     e_x = D.rik(dB, nocc, nmo, naux, dK, want_energy=True)
     extras = {"e_x": e_x}
 
+    # ---- reduced-precision K builds (slicing pass included in the step) -----------
+    K64 = dK.to_host((nmo, nmo))
+    prec = {}
+    dK2 = D.DeviceArray(nmo * nmo)
+    for form, kname in (("ozaki_bf16", "rik_ozaki"), ("bf16x3", "rik_bf16x3")):
+        for _ in range(max(1, args.warmup)):
+            D.rik(dB, nocc, nmo, naux, dK2, precision=form, want_energy=False)
+        D.synchronize()
+        D.kernel_timer_reset()
+        D.kernel_timer(True)
+        ev0.record()
+        for _ in range(args.steps):
+            D.rik(dB, nocc, nmo, naux, dK2, precision=form, want_energy=False)
+        ev1.record()
+        ms = ev0.elapsed_ms(ev1) / args.steps
+        D.kernel_timer(False)
+        kk_ms, kk_n = D.kernel_time(kname)
+        sl_ms, sl_n = D.kernel_time("rik_slice")
+        D.kernel_timer_reset()
+        ex2 = D.rik(dB, nocc, nmo, naux, dK2, precision=form, want_energy=True)
+        K2 = dK2.to_host((nmo, nmo))
+        prec[form] = {
+            "ms_per_step": round(ms, 4),
+            "tflops_fp64_equiv": round(f_k / (ms * 1e-3) / 1e12, 3),
+            "speedup_vs_fp64": round(ms_per_step / ms, 3),
+            "kernel_avg_ms": round(kk_ms / max(kk_n, 1), 4),
+            "slice_avg_ms": round(sl_ms / max(sl_n, 1), 4),
+            "K_rel_frobenius_err": float(np.linalg.norm(K2 - K64) / np.linalg.norm(K64)),
+            "E_x_rel_err": abs(ex2 - e_x) / abs(e_x),
+        }
+        log(f"[bench] RI-K {form}: {ms:.3f} ms/step ({prec[form]['tflops_fp64_equiv']} "
+            f"TFLOP/s FP64-equiv, x{prec[form]['speedup_vs_fp64']} vs FP64), "
+            f"K err {prec[form]['K_rel_frobenius_err']:.2e}")
+    extras["precision_forms"] = prec
+    del dK2
+
     # ---- side measurements: ao2mo, RI-J, unpack ---------------------------------------
     if not args.no_extras:
         D.kernel_timer(True)

@@ -103,10 +103,11 @@ def fill_heff(heff_old, xs, ax, nnew, fresh):
 
 
 def davidson_solve(a_x, x0_1d, hdiag, params: DavidsonParams, precond=None,
-                   max_dx_last_fixed=True) -> DavidsonResult:
+                   max_dx_last_fixed=True, record=None) -> DavidsonResult:
     """davidson_solve (davidson.rs:48-266).  a_x maps a list of vectors to the
     list of A x.  precond(k, r_k, e, x0_k) -> t_k overrides the reference's
-    diagonal preconditioner (which uses e[0] for every root)."""
+    diagonal preconditioner (which uses e[0] for every root).  record(icyc, e,
+    x0, residuals, xs) is called once per cycle after the residuals."""
     tol, maxcyc, lindep, nroots = params.tol, params.maxcyc, params.lindep, params.nroots
     maxspace = params.maxspace + (nroots - 1) * 3
     n = len(x0_1d)
@@ -160,6 +161,8 @@ def davidson_solve(a_x, x0_1d, hdiag, params: DavidsonParams, precond=None,
             dx_norm[k] = float(np.sqrt(xt_k @ xt_k))
             conv[k] = bool(abs(de[k]) < tol and dx_norm[k] < np.sqrt(tol))
         res.history.append((icyc, e.copy(), dx_norm.copy()))
+        if record is not None:
+            record(icyc, e.copy(), x0[:nroots_current], xt, xs)
         res.iterations = icyc + 1
         if all(conv):
             break

@@ -1,0 +1,59 @@
+"""Reduced-precision RI-K builds against the FP64 build and the oracle.
+
+Tolerances stated per form: exponent-aligned BF16 slices (Ozaki-style,
+RT_PREC_OZAKI_BF16) must stay within 1e-8 relative Frobenius error of the
+FP64 K and within 1e-8 relative error in E_x; the unaligned BF16x3 form is
+reported against a looser 1e-4 bar (its BF16 hi/lo split keeps ~16 bits).
+"""
+import numpy as np
+import pytest
+
+from oracle import ri_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def relf(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+@pytest.fixture(scope="module")
+def D():
+    from rest_tensors_amd import device
+    assert device.device_count() >= 1, "no GPU visible"
+    return device
+
+
+def make_B(D, nbas, naux, nocc, seed):
+    dA = D.DeviceArray(nbas * nbas * naux)
+    D.gen_ri3ao(dA, nbas, naux, seed=seed)
+    dC = D.DeviceArray.from_host(O.gen_orthonormal(nbas, seed=seed + 1))
+    dB = D.DeviceArray(nocc * nbas * naux)
+    D.ao2mo_occ(dC, nbas, nbas, nocc, dA, naux, dB)
+    del dA
+    return dB
+
+
+@pytest.mark.parametrize("shape", [(128, 384, 16), (96, 50, 5), (160, 64, 40),
+                                   (512, 2048, 64)])
+def test_ozaki_and_bf16x3_k(D, shape):
+    nbas, naux, nocc = shape
+    dB = make_B(D, nbas, naux, nocc, seed=1000 + nbas)
+    K64, Koz, Kb3 = (D.DeviceArray(nbas * nbas) for _ in range(3))
+    ex64 = D.rik(dB, nocc, nbas, naux, K64, precision="fp64")
+    exoz = D.rik(dB, nocc, nbas, naux, Koz, precision="ozaki_bf16")
+    exb3 = D.rik(dB, nocc, nbas, naux, Kb3, precision="bf16x3")
+    k64 = K64.to_host((nbas, nbas))
+    koz = Koz.to_host((nbas, nbas))
+    kb3 = Kb3.to_host((nbas, nbas))
+    e_oz, e_b3 = relf(koz, k64), relf(kb3, k64)
+    print(f"{shape}: K rel err ozaki {e_oz:.2e}  bf16x3 {e_b3:.2e}; "
+          f"E_x rel err ozaki {abs(exoz - ex64) / abs(ex64):.2e} "
+          f"bf16x3 {abs(exb3 - ex64) / abs(ex64):.2e}")
+    assert e_oz < 1e-8 and abs(exoz - ex64) < 1e-8 * abs(ex64)
+    assert e_b3 < 1e-4
+    assert np.array_equal(koz, koz.T)
+    if nbas <= 160:
+        B = dB.to_host((nocc, nbas, naux))
+        K, ex, _ = O.rik(B)
+        assert relf(koz, K) < 1e-8
