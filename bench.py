@@ -41,10 +41,14 @@ def main():
     ap.add_argument("--naux", type=int, default=4096)
     ap.add_argument("--nocc", type=int, default=128)
     ap.add_argument("--seed", type=int, default=0x5EED0003)
-    ap.add_argument("--cpu-sample", type=int, default=1024,
+    ap.add_argument("--cpu-sample", type=int, default=4096,
                     help="aux slices of the CPU-baseline K sample (0 = skip)")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the ao2mo / RI-J side measurements")
+    ap.add_argument("--batch-mols", type=int, default=64,
+                    help="molecules of the batched workload (0 = skip)")
+    ap.add_argument("--davidson", type=int, default=1,
+                    help="run the Davidson workload (plain vs learned preconditioner)")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="only run one plain device copy of 512 MiB (copy3d kernel: reads "
                          "512 MiB, writes 512 MiB) to calibrate FETCH_SIZE / WRITE_SIZE")
@@ -192,6 +196,12 @@ def main():
         log(f"[bench] ao2mo {ao_ms:.2f} ms, RI-J {j_ms:.2f} ms "
             f"({extras['rij_gbs']} GB/s)")
 
+    del dA
+    if not args.no_extras and args.batch_mols > 0:
+        extras["batched"] = bench_batched(D, O, args)
+    if not args.no_extras and args.davidson:
+        extras["davidson"] = bench_davidson(D, O, args)
+
     # ---- CPU baseline: oracle K on a bounded sample of aux slices ----------------------
     cpu = None
     if args.cpu_sample > 0:
@@ -238,6 +248,123 @@ def main():
         "extras": extras,
     }
     print(json.dumps(out), flush=True)
+
+
+def bench_batched(D, O, args):
+    """Batched workload: molecules with nbas ~ U{64..384} (seeded), naux = 3 nbas,
+    nocc = nbas/8; all stages as grouped launches in one captured HIP graph."""
+    from rest_tensors_amd.batch import Batch
+    rng = np.random.default_rng(0x5EED0004)
+    nbs = rng.integers(64, 385, size=args.batch_mols)
+    mols, flops = [], 0.0
+    for i, nb in enumerate(nbs):
+        nb = int(nb)
+        naux, nocc = 3 * nb, max(1, nb // 8)
+        dA = D.DeviceArray(nb * nb * naux)
+        D.gen_ri3ao(dA, nb, naux, seed=0x5EED0100 + i)
+        mols.append(dict(nbas=nb, naux=naux, nocc=nocc, ri3ao=dA,
+                         C=D.DeviceArray.from_host(O.gen_orthonormal(nb, seed=0x5EED0200 + i)),
+                         D=D.DeviceArray(nb * nb), J=D.DeviceArray(nb * nb),
+                         K=D.DeviceArray(nb * nb), B=D.DeviceArray(nocc * nb * naux)))
+        flops += 4.0 * nocc * nb * nb * naux + float(nb) * (nb + 1) * nocc * naux
+    b = Batch(mols)
+    t0 = time.perf_counter()
+    b.autotune(reps=1)
+    D.synchronize()
+    t_tune = time.perf_counter() - t0
+    s = D.Stream()
+    b.capture(s.handle)
+    b.launch(s.handle)
+    s.synchronize()
+    e0, e1 = D.Event(), D.Event()
+    reps = 5
+    e0.record(s.handle)
+    for _ in range(reps):
+        b.launch(s.handle)
+    e1.record(s.handle)
+    ms = e0.elapsed_ms(e1) / reps
+    ej, ex = b.energies(s.handle)
+    s.destroy()
+    out = {"molecules": len(mols), "nbas_min": int(nbs.min()), "nbas_max": int(nbs.max()),
+           "ms_per_batch_graph": round(ms, 3),
+           "tflops_ao2mo_plus_k": round(flops / (ms * 1e-3) / 1e12, 3),
+           "autotune_s": round(t_tune, 2), "tiles": b.info(),
+           "e_x_sum": float(ex.sum()), "e_j_sum": float(ej.sum())}
+    log(f"[bench] batched {len(mols)} molecules: {ms:.2f} ms per graph replay "
+        f"({out['tflops_ao2mo_plus_k']} TFLOP/s ao2mo+K)")
+    del b, mols
+    return out
+
+
+def bench_davidson(D, O, args):
+    """Davidson workload: A = diag(eps_a - eps_i) + 2 B B^T, B = 0.05 ri3mo(occ,vir,aux)
+    from nbas=512, naux=2048, nocc=64 data (ov = 28672), lowest 8 roots to a 1e-6
+    residual (tol 1e-12); plain diagonal preconditioner vs the MLP trained on
+    plain-Davidson samples of two other seeds."""
+    from oracle import davidson_oracle as DO
+    from rest_tensors_amd import davidson as DV
+    nbas, naux, nocc, nroots = 512, 2048, 64, 8
+    kw = dict(tol=1e-12, maxcyc=1000, maxspace=24)
+
+    def problem(seed):
+        dA = D.DeviceArray(nbas * nbas * naux)
+        D.gen_ri3ao(dA, nbas, naux, seed=seed)
+        dC = D.DeviceArray.from_host(O.gen_orthonormal(nbas, seed=seed + 1))
+        dB = D.DeviceArray(nocc * nbas * naux)
+        D.ao2mo_occ(dC, nbas, nbas, nocc, dA, naux, dB)
+        del dA
+        eps = DO.casida_eps(nbas, nocc, seed=seed + 3)
+        dvec = (eps[nocc:][None, :] - eps[:nocc][:, None]).reshape(-1, order="F")
+        n = dvec.size
+        dd = D.DeviceArray.from_host(dvec)
+        op = DV.CasidaOp(dB, nocc * nbas, n, naux, 0.05, dd, offset=nocc * nocc)
+        hd = D.DeviceArray(n)
+        op.hdiag(hd)
+        hdh = hd.to_host()
+        x0 = np.zeros(n)
+        x0[np.argmin(hdh)] = 1.0
+        return dict(op=op, hd=hd, x0=D.DeviceArray.from_host(x0), n=n, keep=(dB, dd))
+
+    def solve(p, **extra):
+        r = DV.solve(p["op"].matvec_fn, p["op"].handle, p["x0"], p["hd"], p["n"], nroots,
+                     allow_notconv=True, **kw, **extra)
+        return r
+
+    target = problem(0x5EED0002)
+    solve(target)  # warm (kernels, scratch)
+    plain = solve(target)
+    # training data: plain runs on two other seeds, recorded, with targets
+    net = DV.Mlp(seed=1)
+    t0 = time.perf_counter()
+    for seed in (0x5EED1002, 0x5EED2002):
+        p = problem(seed)
+        ref = solve(p)
+        rec = DV.Recorder(p["n"], 4000)
+        solve(p, recorder=rec)
+        ns = rec.count
+        tau = D.DeviceArray(p["n"] * ns)
+        wts = DV.mlp_targets(rec.x, rec.root[:ns], ns, ref.x, p["n"], tau)
+        net.train(rec.r, rec.x, p["hd"], rec.e_k[:ns], rec.e_0[:ns], tau, wts, p["n"], ns,
+                  steps=200, batch=32, lr=3e-3)
+        del rec, tau
+    t_train = time.perf_counter() - t0
+    learned = solve(target, precond="mlp", mlp=net)
+    out = {"n": target["n"], "nroots": nroots, "tol": kw["tol"], "maxspace": kw["maxspace"],
+           "plain": {"seconds": round(plain.stats["seconds"], 4),
+                     "iterations": plain.stats["iterations"],
+                     "matvecs": plain.stats["matvecs"], "converged": all(plain.conv),
+                     "max_residual": plain.stats["max_residual"]},
+           "mlp": {"seconds": round(learned.stats["seconds"], 4),
+                   "iterations": learned.stats["iterations"],
+                   "matvecs": learned.stats["matvecs"], "converged": all(learned.conv),
+                   "max_residual": learned.stats["max_residual"]},
+           "mlp_over_plain_time": round(learned.stats["seconds"] / plain.stats["seconds"], 3),
+           "training_s": round(t_train, 2),
+           "max_eig_diff": float(np.abs(plain.e - learned.e).max())}
+    log(f"[bench] davidson n={target['n']}: plain {out['plain']['seconds']} s "
+        f"({out['plain']['iterations']} it), mlp {out['mlp']['seconds']} s "
+        f"({out['mlp']['iterations']} it), ratio {out['mlp_over_plain_time']}")
+    return out
 
 
 if __name__ == "__main__":

@@ -280,6 +280,39 @@ int rt_mlp_targets_dev(const double* X, const int* root, int ns,
                        const double* Xfinal, int64_t n, double min_overlap,
                        double* tau, double* wts, void* stream);
 
+/* ---- batched pipeline for many molecules (grouped launches + HIP graph) ---- */
+/* Device buffers of one molecule: ri3ao [nbas,nbas,naux], C [nbas,nbas];
+ * outputs D, J, K [nbas,nbas] and B [nocc,nbas,naux] (ri3mo(occ,all,aux)). */
+typedef struct rt_mol_desc {
+  int64_t nbas, naux, nocc;
+  const double* ri3ao;
+  const double* C;
+  double* D;
+  double* J;
+  double* K;
+  double* B;
+} rt_mol_desc;
+
+/* Builds the grouped work lists (one persistent launch per pipeline stage
+ * covering all molecules) and allocates the intermediates. */
+int rt_batch_create(const rt_mol_desc* mols, int nmol, void** handle);
+/* Times every candidate tile edge (64, 128) x persistent grid (1, 2, 4 per
+ * CU) of each grouped GEMM stage on this molecule set and keeps the fastest. */
+int rt_batch_autotune(void* handle, int reps, void* stream);
+/* Runs density, RI-J, ao2mo (occupied rows), RI-K and the energies of every
+ * molecule (8 launches). */
+int rt_batch_run(void* handle, void* stream);
+/* Captures rt_batch_run's launches on `stream` (non-NULL) into a HIP graph;
+ * rt_batch_graph_launch replays it. */
+int rt_batch_graph_capture(void* handle, void* stream);
+int rt_batch_graph_launch(void* handle, void* stream);
+/* Host arrays [nmol]: E_J = 1/2 sum_P gamma_P^2, E_x = -1/2 sum_{n<nocc} K[n,n].
+ * Synchronizes `stream`. */
+int rt_batch_energies(void* handle, double* e_j, double* e_x, void* stream);
+/* JSON text of the chosen tile configuration per stage. */
+int rt_batch_info(void* handle, char* buf, int len);
+int rt_batch_destroy(void* handle);
+
 #ifdef __cplusplus
 }
 #endif

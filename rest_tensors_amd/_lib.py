@@ -85,6 +85,15 @@ RT_SIGNATURES = {
     "rt_mlp_train_dev": (i32, [vp, vp, vp, vp, pf64, pf64, vp, pf64, i64, i32, i32, i32,
                                f64, u64, pf64, vp]),
     "rt_mlp_targets_dev": (i32, [vp, pi32, i32, vp, i64, f64, vp, pf64, vp]),
+    # batched pipeline
+    "rt_batch_create": (i32, [vp, i32, C.POINTER(vp)]),
+    "rt_batch_autotune": (i32, [vp, i32, vp]),
+    "rt_batch_run": (i32, [vp, vp]),
+    "rt_batch_graph_capture": (i32, [vp, vp]),
+    "rt_batch_graph_launch": (i32, [vp, vp]),
+    "rt_batch_energies": (i32, [vp, pf64, pf64, vp]),
+    "rt_batch_info": (i32, [vp, cp, i32]),
+    "rt_batch_destroy": (i32, [vp]),
 }
 
 # the seven drop-in symbols (include/restmatr.h): every argument by reference

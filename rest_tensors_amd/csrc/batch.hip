@@ -1,0 +1,569 @@
+// Batched RI pipeline for many molecules of uneven size (the grouped
+// workload: nbas drawn from [64, 384], naux = 3 nbas, nocc = nbas/8).
+//
+// Every stage of the FP64 pipeline is ONE launch covering all molecules:
+//   1 density      D_m = C_occ C_occ^T          grouped GEMM (persistent)
+//   2 gamma        gamma_m,P = <A_m,P, D_m>      grouped, one block per (m, P)
+//   3 J            J_m = sum_P A_m,P gamma_m,P   grouped, one block per (m, j, seg)
+//   4 ao2mo s1     X^T = A_P^T C_occ             grouped GEMM
+//   5 ao2mo s2     B_P = (X^T)^T C               grouped GEMM
+//   6 K partials   sum_P B_P^T B_P (upper tiles) grouped GEMM, batch-reducing
+//   7 K reduce     slot sums + mirror            grouped, one block per (m, 32-tile)
+//   8 energies     E_J, E_x per molecule         one block per molecule
+// The eight launches are captured once into a HIP graph and replayed.  The
+// grouped GEMMs walk a device work list (descriptor + prefix sums); their
+// tile edge (64 or 128) and persistent grid size come from an autotuner that
+// times every candidate on the actual molecule set and keeps the fastest.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+#include "gemm.hpp"
+#include "mfma_f64.hpp"
+#include "timer.hpp"
+
+namespace rta {
+
+struct MolDev {
+  const double* A;
+  const double* C;
+  double *D, *J, *K, *B;
+  double *X, *gamma, *part;
+  int64_t nb, naux, nocc;
+  int64_t nchunks;  // K partial slots
+};
+
+__device__ __forceinline__ int find_mol(const int64_t* __restrict__ offs, int nmol,
+                                        int64_t g) {
+  int lo = 0, hi = nmol;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (offs[mid] <= g) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ double b_wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// one block per (molecule, aux slice)
+__global__ __launch_bounds__(256) void batch_gamma_kernel(
+    const MolDev* __restrict__ mols, const int64_t* __restrict__ offs, int nmol) {
+  __shared__ double red[4];
+  const int m = find_mol(offs, nmol, blockIdx.x);
+  const MolDev M = mols[m];
+  const int64_t P = blockIdx.x - offs[m];
+  const int64_t nb = M.nb;
+  const double* Ap = M.A + P * nb * nb;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  double acc = 0.0;
+  for (int64_t j = wave; j < nb; j += 4) {
+    const double* col = Ap + j * nb;
+    const double* dcol = M.D + j * nb;
+    for (int64_t i = lane; i <= j; i += 64) acc += (i == j ? 1.0 : 2.0) * col[i] * dcol[i];
+  }
+  acc = b_wave_sum(acc);
+  if (lane == 0) red[wave] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) M.gamma[P] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+// one block per (molecule, column j, 64-row segment)
+__global__ __launch_bounds__(256) void batch_j_kernel(const MolDev* __restrict__ mols,
+                                                      const int64_t* __restrict__ offs,
+                                                      int nmol) {
+  __shared__ double red[4][64];
+  const int m = find_mol(offs, nmol, blockIdx.x);
+  const MolDev M = mols[m];
+  const int64_t nb = M.nb, nseg = (nb + 63) / 64;
+  const int64_t loc = blockIdx.x - offs[m];
+  const int64_t j = loc / nseg, i0 = (loc % nseg) * 64;
+  if (i0 > j) return;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int64_t i = i0 + lane, n2 = nb * nb;
+  double acc = 0.0;
+  if (i <= j) {
+    const double* p = M.A + i + j * nb;
+    for (int64_t P = wave; P < M.naux; P += 4) acc += p[P * n2] * M.gamma[P];
+  }
+  red[wave][lane] = acc;
+  __syncthreads();
+  if (wave == 0 && i <= j) {
+    const double v = (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
+    M.J[i + j * nb] = v;
+    if (i != j) M.J[j + i * nb] = v;
+  }
+}
+
+// one block per (molecule, upper 32x32 tile): K = sum of partial slots, mirrored
+__global__ __launch_bounds__(256) void batch_kreduce_kernel(
+    const MolDev* __restrict__ mols, const int64_t* __restrict__ offs, int nmol) {
+  __shared__ double T[32][33];
+  const int m = find_mol(offs, nmol, blockIdx.x);
+  const MolDev M = mols[m];
+  int tm, tn;
+  upper_tile(blockIdx.x - offs[m], tm, tn);
+  const int64_t nb = M.nb, slot = nb * nb;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int64_t r = (int64_t)tm * 32 + tx, c = (int64_t)tn * 32 + ty + 8 * q;
+    double s = 0.0;
+    if (r < nb && c < nb && r <= c) {
+      const double* p = M.part + r + c * nb;
+      for (int64_t k = 0; k < M.nchunks; ++k) s += p[k * slot];
+      M.K[r + c * nb] = s;
+    }
+    T[ty + 8 * q][tx] = s;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int64_t c = (int64_t)tn * 32 + tx, r = (int64_t)tm * 32 + ty + 8 * q;
+    if (r < nb && c < nb && r < c) M.K[c + r * nb] = T[tx][ty + 8 * q];
+  }
+}
+
+// one block per molecule: energies[2m] = E_J = 1/2 sum gamma^2,
+// energies[2m+1] = E_x = -1/2 sum_{n<nocc} K[n,n]
+__global__ __launch_bounds__(256) void batch_energy_kernel(const MolDev* __restrict__ mols,
+                                                           double* __restrict__ energies) {
+  __shared__ double red[2][4];
+  const MolDev M = mols[blockIdx.x];
+  double a = 0.0, b = 0.0;
+  for (int64_t P = threadIdx.x; P < M.naux; P += 256) a += M.gamma[P] * M.gamma[P];
+  for (int64_t n = threadIdx.x; n < M.nocc; n += 256) b += M.K[n + n * M.nb];
+  a = b_wave_sum(a);
+  b = b_wave_sum(b);
+  if ((threadIdx.x & 63) == 0) {
+    red[0][threadIdx.x >> 6] = a;
+    red[1][threadIdx.x >> 6] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    energies[2 * blockIdx.x] = 0.5 * ((red[0][0] + red[0][1]) + (red[0][2] + red[0][3]));
+    energies[2 * blockIdx.x + 1] =
+        -0.5 * ((red[1][0] + red[1][1]) + (red[1][2] + red[1][3]));
+  }
+}
+
+// ---- host side ------------------------------------------------------------------
+struct GroupCfg {
+  int tile = 64;   // 64 or 128
+  int grid_mul = 2;  // persistent workgroups per CU
+};
+
+struct GroupedStage {
+  bool ta = true, tb = false;
+  GroupCfg cfg;
+  std::vector<GemmParams> descs;
+  GemmParams* d_descs = nullptr;
+  int64_t* d_offs = nullptr;
+  int64_t total = 0;
+};
+
+struct Batch {
+  int nmol = 0;
+  std::vector<MolDev> mols;
+  MolDev* d_mols = nullptr;
+  int64_t *d_off_gamma = nullptr, *d_off_j = nullptr, *d_off_kred = nullptr;
+  int64_t n_gamma = 0, n_j = 0, n_kred = 0;
+  double* energies = nullptr;  // device [2*nmol]
+  std::vector<void*> owned;    // device allocations of the handle
+  GroupedStage st[4];          // density, ao2mo s1, ao2mo s2, K partials
+  double tune_ms[4] = {0, 0, 0, 0};
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+static int dalloc(Batch* b, size_t bytes, void** p) {
+  hipError_t e = hipMalloc(p, std::max<size_t>(bytes, 16));
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(RT_ERR_NOMEM, "batch: allocation of %zu bytes failed", bytes);
+  }
+  b->owned.push_back(*p);
+  return RT_OK;
+}
+
+static GemmParams make_params(const GemmDesc& d, int tile, bool reduce, int64_t bchunk) {
+  GemmParams g{};
+  g.A = d.A;
+  g.B = d.B;
+  g.C = d.C;
+  g.M = d.M;
+  g.N = d.N;
+  g.K = d.K;
+  g.lda = d.lda;
+  g.ldb = d.ldb;
+  g.sA = d.sA;
+  g.sB = d.sB;
+  g.c_rs = d.c_rs;
+  g.c_cs = d.c_cs;
+  g.sC = d.sC;
+  g.alpha = d.alpha;
+  g.beta = d.beta;
+  g.nbatch = d.batch;
+  g.tiles_m = (int)((d.M + tile - 1) / tile);
+  g.tiles_n = (int)((d.N + tile - 1) / tile);
+  if (reduce) {
+    g.bchunk = bchunk;
+    g.nchunks = (d.batch + bchunk - 1) / bchunk;
+    g.ntiles = (int64_t)g.tiles_m * (g.tiles_m + 1) / 2;
+    g.reduce = 1;
+    g.upper = 1;
+  } else {
+    g.bchunk = 1;
+    g.nchunks = d.batch;
+    g.ntiles = (int64_t)g.tiles_m * g.tiles_n;
+  }
+  g.nwork = g.ntiles * g.nchunks;
+  return g;
+}
+
+// Descriptors of stage s for tile edge `tile`.  For the K stage the batch
+// chunking is fixed by the partial-slot layout chosen at creation.
+static void build_stage(Batch* b, int s, int tile) {
+  GroupedStage& S = b->st[s];
+  S.descs.clear();
+  for (const MolDev& m : b->mols) {
+    GemmDesc d;
+    d.alpha = 1.0;
+    if (s == 0) {  // D = C_occ C_occ^T
+      d.ta = false;
+      d.tb = true;
+      d.M = d.N = m.nb;
+      d.K = m.nocc;
+      d.A = m.C;
+      d.lda = m.nb;
+      d.B = m.C;
+      d.ldb = m.nb;
+      d.C = m.D;
+      d.c_cs = m.nb;
+      S.descs.push_back(make_params(d, tile, false, 1));
+    } else if (s == 1) {  // X^T_P = A_P^T C_occ
+      d.M = m.nb;
+      d.N = m.nocc;
+      d.K = m.nb;
+      d.A = m.A;
+      d.lda = m.nb;
+      d.sA = m.nb * m.nb;
+      d.B = m.C;
+      d.ldb = m.nb;
+      d.C = m.X;
+      d.c_cs = m.nb;
+      d.sC = m.nb * m.nocc;
+      d.batch = m.naux;
+      S.descs.push_back(make_params(d, tile, false, 1));
+    } else if (s == 2) {  // B_P = (X^T_P)^T C
+      d.M = m.nocc;
+      d.N = m.nb;
+      d.K = m.nb;
+      d.A = m.X;
+      d.lda = m.nb;
+      d.sA = m.nb * m.nocc;
+      d.B = m.C;
+      d.ldb = m.nb;
+      d.C = m.B;
+      d.c_cs = m.nocc;
+      d.sC = m.nocc * m.nb;
+      d.batch = m.naux;
+      S.descs.push_back(make_params(d, tile, false, 1));
+    } else {  // K partials: sum_P B_P^T B_P, upper tiles
+      d.M = d.N = m.nb;
+      d.K = m.nocc;
+      d.A = m.B;
+      d.lda = m.nocc;
+      d.sA = m.nocc * m.nb;
+      d.B = m.B;
+      d.ldb = m.nocc;
+      d.sB = m.nocc * m.nb;
+      d.C = m.part;
+      d.c_rs = 1;
+      d.c_cs = m.nb;
+      d.sC = m.nb * m.nb;
+      d.batch = m.naux;
+      const int64_t bchunk = (m.naux + m.nchunks - 1) / m.nchunks;
+      GemmParams g = make_params(d, tile, true, bchunk);
+      S.descs.push_back(g);
+    }
+  }
+  S.cfg.tile = tile;
+  S.ta = (s != 0);
+  S.tb = (s == 0);
+}
+
+static int upload_stage(Batch* b, int s) {
+  GroupedStage& S = b->st[s];
+  std::vector<int64_t> offs(S.descs.size() + 1, 0);
+  for (size_t i = 0; i < S.descs.size(); ++i) offs[i + 1] = offs[i] + S.descs[i].nwork;
+  S.total = offs.back();
+  if (!S.d_descs) {
+    RTA_TRY(dalloc(b, sizeof(GemmParams) * S.descs.size(), (void**)&S.d_descs));
+    RTA_TRY(dalloc(b, sizeof(int64_t) * offs.size(), (void**)&S.d_offs));
+  }
+  RTA_HIP(hipMemcpy(S.d_descs, S.descs.data(), sizeof(GemmParams) * S.descs.size(),
+                    hipMemcpyHostToDevice));
+  RTA_HIP(hipMemcpy(S.d_offs, offs.data(), sizeof(int64_t) * offs.size(),
+                    hipMemcpyHostToDevice));
+  return RT_OK;
+}
+
+template <int T, bool TA, bool TB>
+static void launch_grouped_t(const GroupedStage& S, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL((dgemm_grouped_kernel<T, T, 16, TA, TB>), dim3(grid), dim3(256), 0,
+                     s, S.d_descs, S.d_offs, (int)S.descs.size());
+}
+
+static int launch_grouped(const GroupedStage& S, hipStream_t s) {
+  const int64_t want = (int64_t)device_cu_count() * S.cfg.grid_mul;
+  const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(want, S.total));
+  if (S.total == 0) return RT_OK;
+  if (S.cfg.tile == 128) {
+    if (S.ta) launch_grouped_t<128, true, false>(S, grid, s);
+    else launch_grouped_t<128, false, true>(S, grid, s);
+  } else {
+    if (S.ta) launch_grouped_t<64, true, false>(S, grid, s);
+    else launch_grouped_t<64, false, true>(S, grid, s);
+  }
+  RTA_HIP(hipGetLastError());
+  return RT_OK;
+}
+
+static int run_stages(Batch* b, hipStream_t s) {
+  RTA_TRY(launch_grouped(b->st[0], s));  // density
+  if (b->n_gamma > 0)
+    hipLaunchKernelGGL(batch_gamma_kernel, dim3((unsigned)b->n_gamma), dim3(256), 0, s,
+                       b->d_mols, b->d_off_gamma, b->nmol);
+  if (b->n_j > 0)
+    hipLaunchKernelGGL(batch_j_kernel, dim3((unsigned)b->n_j), dim3(256), 0, s, b->d_mols,
+                       b->d_off_j, b->nmol);
+  RTA_HIP(hipGetLastError());
+  RTA_TRY(launch_grouped(b->st[1], s));  // ao2mo stage 1
+  RTA_TRY(launch_grouped(b->st[2], s));  // ao2mo stage 2
+  RTA_TRY(launch_grouped(b->st[3], s));  // K partials
+  if (b->n_kred > 0)
+    hipLaunchKernelGGL(batch_kreduce_kernel, dim3((unsigned)b->n_kred), dim3(256), 0, s,
+                       b->d_mols, b->d_off_kred, b->nmol);
+  hipLaunchKernelGGL(batch_energy_kernel, dim3(b->nmol), dim3(256), 0, s, b->d_mols,
+                     b->energies);
+  RTA_HIP(hipGetLastError());
+  return RT_OK;
+}
+
+static double time_stage(Batch* b, int s, hipStream_t stream, int reps) {
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return 1e30;
+  if (launch_grouped(b->st[s], stream) != RT_OK) return 1e30;  // warm
+  (void)hipEventRecord(e0, stream);
+  for (int r = 0; r < reps; ++r)
+    if (launch_grouped(b->st[s], stream) != RT_OK) return 1e30;
+  (void)hipEventRecord(e1, stream);
+  (void)hipEventSynchronize(e1);
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return ms / reps;
+}
+
+}  // namespace rta
+
+using namespace rta;
+
+extern "C" int rt_batch_create(const rt_mol_desc* mols, int nmol, void** handle) {
+  RTA_CHECK(mols && handle && nmol > 0, "bad arguments");
+  Batch* b = new Batch();
+  b->nmol = nmol;
+  auto fail_cleanup = [&](int st) {
+    for (void* p : b->owned) (void)hipFree(p);
+    delete b;
+    return st;
+  };
+  int64_t ng = 0, nj = 0, nk = 0;
+  std::vector<int64_t> og(1, 0), oj(1, 0), ok(1, 0);
+  for (int i = 0; i < nmol; ++i) {
+    const rt_mol_desc& m = mols[i];
+    if (!(m.nbas > 0 && m.naux > 0 && m.nocc > 0 && m.nocc <= m.nbas && m.ri3ao && m.C &&
+          m.D && m.J && m.K && m.B))
+      return fail_cleanup(fail(RT_ERR_ARG, "molecule %d: bad shape or NULL buffer", i));
+    MolDev d{};
+    d.A = m.ri3ao;
+    d.C = m.C;
+    d.D = m.D;
+    d.J = m.J;
+    d.K = m.K;
+    d.B = m.B;
+    d.nb = m.nbas;
+    d.naux = m.naux;
+    d.nocc = m.nocc;
+    d.nchunks = std::max<int64_t>(1, std::min<int64_t>(m.naux, 8));
+    int st = dalloc(b, sizeof(double) * m.nbas * m.nocc * m.naux, (void**)&d.X);
+    if (st == RT_OK) st = dalloc(b, sizeof(double) * m.naux, (void**)&d.gamma);
+    if (st == RT_OK)
+      st = dalloc(b, sizeof(double) * d.nchunks * m.nbas * m.nbas, (void**)&d.part);
+    if (st != RT_OK) return fail_cleanup(st);
+    b->mols.push_back(d);
+    const int64_t t32 = (m.nbas + 31) / 32;
+    ng += m.naux;
+    nj += m.nbas * ((m.nbas + 63) / 64);
+    nk += t32 * (t32 + 1) / 2;
+    og.push_back(ng);
+    oj.push_back(nj);
+    ok.push_back(nk);
+  }
+  b->n_gamma = ng;
+  b->n_j = nj;
+  b->n_kred = nk;
+  int st = dalloc(b, sizeof(MolDev) * nmol, (void**)&b->d_mols);
+  if (st == RT_OK) st = dalloc(b, sizeof(int64_t) * og.size(), (void**)&b->d_off_gamma);
+  if (st == RT_OK) st = dalloc(b, sizeof(int64_t) * oj.size(), (void**)&b->d_off_j);
+  if (st == RT_OK) st = dalloc(b, sizeof(int64_t) * ok.size(), (void**)&b->d_off_kred);
+  if (st == RT_OK) st = dalloc(b, sizeof(double) * 2 * nmol, (void**)&b->energies);
+  if (st != RT_OK) return fail_cleanup(st);
+  if (hipMemcpy(b->d_mols, b->mols.data(), sizeof(MolDev) * nmol, hipMemcpyHostToDevice) !=
+          hipSuccess ||
+      hipMemcpy(b->d_off_gamma, og.data(), sizeof(int64_t) * og.size(),
+                hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(b->d_off_j, oj.data(), sizeof(int64_t) * oj.size(), hipMemcpyHostToDevice) !=
+          hipSuccess ||
+      hipMemcpy(b->d_off_kred, ok.data(), sizeof(int64_t) * ok.size(),
+                hipMemcpyHostToDevice) != hipSuccess)
+    return fail_cleanup(fail(RT_ERR_HIP, "batch: descriptor upload failed"));
+  for (int s = 0; s < 4; ++s) {
+    build_stage(b, s, 64);
+    st = upload_stage(b, s);
+    if (st != RT_OK) return fail_cleanup(st);
+  }
+  *handle = b;
+  return RT_OK;
+}
+
+extern "C" int rt_batch_autotune(void* h, int reps, void* stream) {
+  RTA_CHECK(h, "NULL handle");
+  Batch* b = (Batch*)h;
+  hipStream_t s = (hipStream_t)stream;
+  reps = std::max(1, reps);
+  // stages 1 and 2 read D / X written by earlier stages: run once so every
+  // candidate sees valid inputs (results are recomputed by rt_batch_run)
+  RTA_TRY(run_stages(b, s));
+  RTA_HIP(hipStreamSynchronize(s));
+  for (int stg = 0; stg < 4; ++stg) {
+    double best = 1e30;
+    GroupCfg bestc;
+    for (int tile : {64, 128}) {
+      build_stage(b, stg, tile);
+      RTA_TRY(upload_stage(b, stg));
+      for (int gm : {1, 2, 4}) {
+        b->st[stg].cfg.grid_mul = gm;
+        const double ms = time_stage(b, stg, s, reps);
+        if (ms < best) {
+          best = ms;
+          bestc = b->st[stg].cfg;
+        }
+      }
+    }
+    build_stage(b, stg, bestc.tile);
+    RTA_TRY(upload_stage(b, stg));
+    b->st[stg].cfg = bestc;
+    b->tune_ms[stg] = best;
+  }
+  if (b->exec) {  // the graph baked in the old descriptors' launch shapes
+    (void)hipGraphExecDestroy(b->exec);
+    (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+  }
+  return RT_OK;
+}
+
+extern "C" int rt_batch_run(void* h, void* stream) {
+  RTA_CHECK(h, "NULL handle");
+  KernelTimer kt("batch_run", (hipStream_t)stream);
+  return run_stages((Batch*)h, (hipStream_t)stream);
+}
+
+extern "C" int rt_batch_graph_capture(void* h, void* stream) {
+  RTA_CHECK(h && stream, "graph capture needs a handle and a non-null stream");
+  Batch* b = (Batch*)h;
+  hipStream_t s = (hipStream_t)stream;
+  if (b->exec) {
+    (void)hipGraphExecDestroy(b->exec);
+    (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+  }
+  // run_stages records no timing events, so the graph holds kernel nodes only
+  RTA_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  int st = run_stages(b, s);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(s, &g);
+  if (st != RT_OK) {
+    if (g) (void)hipGraphDestroy(g);
+    return st;
+  }
+  if (e != hipSuccess)
+    return fail(RT_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+  RTA_HIP(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
+  b->graph = g;
+  return RT_OK;
+}
+
+extern "C" int rt_batch_graph_launch(void* h, void* stream) {
+  RTA_CHECK(h, "NULL handle");
+  Batch* b = (Batch*)h;
+  RTA_CHECK(b->exec, "no captured graph (call rt_batch_graph_capture first)");
+  KernelTimer kt("batch_graph", (hipStream_t)stream);
+  RTA_HIP(hipGraphLaunch(b->exec, (hipStream_t)stream));
+  return RT_OK;
+}
+
+extern "C" int rt_batch_energies(void* h, double* e_j, double* e_x, void* stream) {
+  RTA_CHECK(h && e_j && e_x, "NULL argument");
+  Batch* b = (Batch*)h;
+  std::vector<double> e(2 * b->nmol);
+  RTA_HIP(hipMemcpyAsync(e.data(), b->energies, sizeof(double) * e.size(),
+                         hipMemcpyDeviceToHost, (hipStream_t)stream));
+  RTA_HIP(hipStreamSynchronize((hipStream_t)stream));
+  for (int i = 0; i < b->nmol; ++i) {
+    e_j[i] = e[2 * i];
+    e_x[i] = e[2 * i + 1];
+  }
+  return RT_OK;
+}
+
+extern "C" int rt_batch_info(void* h, char* buf, int len) {
+  RTA_CHECK(h && buf && len > 0, "NULL argument");
+  Batch* b = (Batch*)h;
+  static const char* names[4] = {"density", "ao2mo_s1", "ao2mo_s2", "k_partials"};
+  std::string s = "{";
+  for (int i = 0; i < 4; ++i) {
+    char t[160];
+    snprintf(t, sizeof t, "%s\"%s\": {\"tile\": %d, \"grid_mul\": %d, \"tuned_ms\": %.4f}",
+             i ? ", " : "", names[i], b->st[i].cfg.tile, b->st[i].cfg.grid_mul,
+             b->tune_ms[i]);
+    s += t;
+  }
+  s += "}";
+  snprintf(buf, len, "%s", s.c_str());
+  return RT_OK;
+}
+
+extern "C" int rt_batch_destroy(void* h) {
+  if (!h) return RT_OK;
+  Batch* b = (Batch*)h;
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  for (void* p : b->owned) (void)hipFree(p);
+  delete b;
+  return RT_OK;
+}

@@ -113,18 +113,20 @@ __device__ __forceinline__ void store_tile(double* __restrict__ S,
   }
 }
 
+constexpr int kGemmPad = 1;
+
+// One work item w (tile x batch chunk) of a GEMM; every thread of the 256-wide
+// workgroup must call it (it synchronizes).  As/Bs: LDS of
+// BK*(BM+1) and BK*(BN+1) doubles.
 template <int BM, int BN, int BK, bool TA, bool TB>
-__global__ __launch_bounds__(256) void dgemm_mfma_kernel(GemmParams g) {
-  constexpr int PAD = 1;
+__device__ __forceinline__ void dgemm_tile(const GemmParams& g, int64_t w,
+                                           double* __restrict__ As,
+                                           double* __restrict__ Bs) {
+  constexpr int PAD = kGemmPad;
   constexpr int WM = BM / 2, WN = BN / 2;
   constexpr int FM = WM / 16, FN = WN / 16;
   static_assert(BK % 4 == 0 && (BM * BK) % 256 == 0 && (BN * BK) % 256 == 0,
                 "tile shape");
-  __shared__ double As[BK * (BM + PAD)];
-  __shared__ double Bs[BK * (BN + PAD)];
-
-  const int64_t w = xcd_remap(blockIdx.x, gridDim.x);
-  if (w >= g.nwork) return;
   const int64_t chunk = w / g.ntiles;
   const int64_t tid = w % g.ntiles;
   int tm, tn;
@@ -208,6 +210,38 @@ __global__ __launch_bounds__(256) void dgemm_mfma_kernel(GemmParams g) {
         }
       }
     }
+  }
+}
+
+template <int BM, int BN, int BK, bool TA, bool TB>
+__global__ __launch_bounds__(256) void dgemm_mfma_kernel(GemmParams g) {
+  __shared__ double As[BK * (BM + kGemmPad)];
+  __shared__ double Bs[BK * (BN + kGemmPad)];
+  const int64_t w = xcd_remap(blockIdx.x, gridDim.x);
+  if (w >= g.nwork) return;
+  dgemm_tile<BM, BN, BK, TA, TB>(g, w, As, Bs);
+}
+
+// Persistent grouped GEMM: descs[d] is one (batched or batch-reducing) GEMM
+// with descs[d].nwork work items; offs holds the exclusive prefix sums of
+// nwork (offs[ndesc] = total).  Workgroups stride over the global work list.
+template <int BM, int BN, int BK, bool TA, bool TB>
+__global__ __launch_bounds__(256) void dgemm_grouped_kernel(
+    const GemmParams* __restrict__ descs, const int64_t* __restrict__ offs,
+    int ndesc) {
+  __shared__ double As[BK * (BM + kGemmPad)];
+  __shared__ double Bs[BK * (BN + kGemmPad)];
+  const int64_t total = offs[ndesc];
+  for (int64_t gw = blockIdx.x; gw < total; gw += gridDim.x) {
+    int lo = 0, hi = ndesc;
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (offs[mid] <= gw) lo = mid;
+      else hi = mid;
+    }
+    const GemmParams g = descs[lo];
+    dgemm_tile<BM, BN, BK, TA, TB>(g, gw - offs[lo], As, Bs);
+    __syncthreads();  // LDS is reused by the next work item
   }
 }
 
