@@ -138,7 +138,8 @@ def main():
     K64 = dK.to_host((nmo, nmo))
     prec = {}
     dK2 = D.DeviceArray(nmo * nmo)
-    for form, kname in (("ozaki_bf16", "rik_ozaki"), ("bf16x3", "rik_bf16x3")):
+    for form, kname in (("ozaki_int8", "rik_oz_i8"), ("ozaki_bf16", "rik_ozaki"),
+                        ("bf16x3", "rik_bf16x3")):
         for _ in range(max(1, args.warmup)):
             D.rik(dB, nocc, nmo, naux, dK2, precision=form, want_energy=False)
         D.synchronize()

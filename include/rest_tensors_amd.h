@@ -36,6 +36,9 @@ extern "C" {
 #define RT_PREC_BF16X3 2      /* hi/lo BF16 split, 3 BF16-MFMA products    */
 #define RT_PREC_OZAKI_BF16 3  /* exponent-aligned BF16 slices, exact FP32
                                  partial sums, FP64 accumulation           */
+#define RT_PREC_OZAKI_INT8 4  /* Ozaki-scheme INT8 slices (4 per value,
+                                 exponent per row and aux group), exact
+                                 int32 level sums, FP64 folding            */
 
 /* ---- runtime plumbing -------------------------------------------------- */
 const char* rt_last_error(void);
@@ -138,6 +141,9 @@ int rt_rij_dev(const double* ri3ao, int64_t nbas, int64_t naux,
  * Synchronizes `stream` when e_x != NULL. */
 int rt_rik_dev(const double* B, int64_t nocc, int64_t nmo, int64_t naux,
                double* K, double* e_x, int precision, void* stream);
+/* Aux slices that share one exponent per row in RT_PREC_OZAKI_INT8 on the
+ * current device (the slicing is reproducible from this number). */
+int64_t rt_ozaki_i8_group_size(int64_t nocc, int64_t nmo, int64_t naux);
 /* E_x by the second route: -1/2 sum_P sum_{m,n<nocc} B[m,n,P]^2, FP64
  * accumulation (host result; synchronizes). */
 int rt_exchange_energy_dev(const double* B, int64_t nocc, int64_t nmo,

@@ -174,6 +174,50 @@ def rik(B: np.ndarray):
     return np.asfortranarray(K), e_x, e_x2
 
 
+def ozaki_i8_slices(B: np.ndarray, gp: int):
+    """Int8 Ozaki slicing of B [nocc, nmo, naux] with one exponent per row i
+    and aux group of gp slices: returns sigma [4, nocc, nmo, naux] (integer
+    valued float64, |sigma| <= 127) and E [ngroups, nmo] with
+    B = 2^E / 127 * sum_s sigma_s 254^-s up to 2^E / 127 * 254^-3 / 2.
+    No reference counterpart: it restates the reduced-precision K form's
+    slicing step by step (same FP64 operations, round-half-even) so that the
+    device result can be checked to the last bits."""
+    nocc, nmo, naux = B.shape
+    ng = (naux + gp - 1) // gp
+    sig = np.zeros((4, nocc, nmo, naux))
+    E = np.zeros((ng, nmo), dtype=np.int64)
+    for g in range(ng):
+        blk = B[:, :, g * gp:(g + 1) * gp]
+        mx = np.abs(blk).max(axis=(0, 2)) if blk.size else np.zeros(nmo)
+        e = np.where(mx > 0, np.frexp(mx)[1], 0).astype(np.int64)
+        E[g] = e
+        u = np.ldexp(blk, -e[None, :, None])
+        x = u * 127.0
+        for s in range(4):
+            sg = np.rint(x)
+            sig[s, :, :, g * gp:(g + 1) * gp] = sg
+            x = (x - sg) * 254.0
+    return sig, E
+
+
+def rik_ozaki_i8(B: np.ndarray, gp: int) -> np.ndarray:
+    """K from the int8 slices: per group, the exact level sums
+    L_q = sum_{s+t=q} sigma_s^T sigma_t (q <= 3; integer products summed in
+    float64, exact below 2^53) folded as 2^(E_i+E_j) / 127^2 sum_q 254^-q L_q."""
+    nocc, nmo, naux = B.shape
+    sig, E = ozaki_i8_slices(B, gp)
+    K = np.zeros((nmo, nmo))
+    for g in range(E.shape[0]):
+        S = [sig[s, :, :, g * gp:(g + 1) * gp].transpose(2, 0, 1).reshape(-1, nmo)
+             for s in range(4)]
+        L = [sum(S[s].T @ S[q - s] for s in range(q + 1)) for q in range(4)]
+        v = L[3]
+        for q in (2, 1, 0):
+            v = L[q] + v * (1.0 / 254.0)
+        K += np.ldexp(v * (1.0 / 16129.0), E[g][:, None] + E[g][None, :])
+    return np.asfortranarray(K)
+
+
 # ---- packed upper ---------------------------------------------------------------
 def upper_dim(packed_len: int) -> int:
     """Dimension n with n(n+1)/2 == packed_len, else -1 (integer isqrt rather

@@ -57,6 +57,7 @@ RT_SIGNATURES = {
     "rt_rij_dev": (i32, [vp, i64, i64, vp, vp, vp, pf64, vp]),
     "rt_rik_dev": (i32, [vp, i64, i64, i64, vp, pf64, i32, vp]),
     "rt_exchange_energy_dev": (i32, [vp, i64, i64, i64, pf64, vp]),
+    "rt_ozaki_i8_group_size": (i64, [i64, i64, i64]),
     "rt_upper_unpack_dev": (i32, [vp, i64, vp, vp]),
     "rt_upper_pack_dev": (i32, [vp, i64, vp, vp]),
     "rt_rifull_to_matfull_symm_dev": (i32, [vp, i64, i64, vp, vp]),

@@ -293,9 +293,11 @@ static int launch_sliced(const SlicedParams& p, hipStream_t s) {
   return RT_OK;
 }
 
+// out (upper, mirrored) = [out +] sum of the slots' upper triangles
 __global__ __launch_bounds__(256) void reduce_sym_kernel2(const double* __restrict__ part,
                                                           int64_t nslot, int64_t M,
-                                                          double* __restrict__ out) {
+                                                          double* __restrict__ out,
+                                                          int accumulate) {
   __shared__ double T[32][33];
   int tm, tn;
   upper_tile(blockIdx.x, tm, tn);
@@ -308,6 +310,7 @@ __global__ __launch_bounds__(256) void reduce_sym_kernel2(const double* __restri
     if (m < M && n < M && m <= n) {
       const double* p = part + m + n * M;
       for (int64_t k = 0; k < nslot; ++k) s += p[k * slot];
+      if (accumulate) s += out[m + n * M];
       out[m + n * M] = s;
     }
     T[ty + 8 * q][tx] = s;
@@ -318,6 +321,16 @@ __global__ __launch_bounds__(256) void reduce_sym_kernel2(const double* __restri
     int64_t n = (int64_t)tn * 32 + tx, m = (int64_t)tm * 32 + ty + 8 * q;
     if (m < M && n < M && m < n) out[n + m * M] = T[tx][ty + 8 * q];
   }
+}
+
+int reduce_partials_sym(const double* part, int64_t nslot, int64_t M, double* K,
+                        bool accumulate, hipStream_t s) {
+  const int64_t t32 = (M + 31) / 32;
+  KernelTimer kt("reduce_sym", s);
+  hipLaunchKernelGGL(reduce_sym_kernel2, dim3((unsigned)(t32 * (t32 + 1) / 2)), dim3(256),
+                     0, s, part, nslot, M, K, accumulate ? 1 : 0);
+  RTA_HIP(hipGetLastError());
+  return RT_OK;
 }
 
 int rik_sliced(const uint16_t* S, const int32_t* E, int64_t nocc, int64_t nmo,
@@ -348,7 +361,7 @@ int rik_sliced(const uint16_t* S, const int32_t* E, int64_t nocc, int64_t nmo,
   const int64_t t32 = (nmo + 31) / 32;
   KernelTimer kt("reduce_sym", s);
   hipLaunchKernelGGL(reduce_sym_kernel2, dim3((unsigned)(t32 * (t32 + 1) / 2)), dim3(256),
-                     0, s, part, nchunks, nmo, K);
+                     0, s, part, nchunks, nmo, K, 0);
   RTA_HIP(hipGetLastError());
   return RT_OK;
 }

@@ -28,4 +28,9 @@ int slice_b(const double* B, int64_t nocc, int64_t nmo, int64_t naux, int ns,
 int rik_sliced(const uint16_t* S, const int32_t* E, int64_t nocc, int64_t nmo,
                int64_t naux, int ns, bool aligned, double* K, hipStream_t s);
 
+// K[m,n] = K[n,m] = [K[m,n] +] sum_{k<nslot} part[k*M*M + m + n*M] for m <= n
+// (fixed order), reading only the upper triangle of each M x M slot.
+int reduce_partials_sym(const double* part, int64_t nslot, int64_t M, double* K,
+                        bool accumulate, hipStream_t s);
+
 }  // namespace rta

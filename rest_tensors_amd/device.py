@@ -11,7 +11,7 @@ import numpy as np
 
 from ._lib import check, lib
 
-PREC = {"fp64": 0, "split_fp32": 1, "bf16x3": 2, "ozaki_bf16": 3}
+PREC = {"fp64": 0, "split_fp32": 1, "bf16x3": 2, "ozaki_bf16": 3, "ozaki_int8": 4}
 
 
 def _as_c_vector(a: np.ndarray, dtype=np.float64) -> np.ndarray:

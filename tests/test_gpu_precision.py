@@ -57,3 +57,30 @@ def test_ozaki_and_bf16x3_k(D, shape):
         B = dB.to_host((nocc, nbas, naux))
         K, ex, _ = O.rik(B)
         assert relf(koz, K) < 1e-8
+
+
+@pytest.mark.parametrize("shape", [(128, 384, 16), (96, 50, 5), (160, 64, 40),
+                                   (200, 300, 70), (512, 2048, 64)])
+def test_ozaki_int8_k(D, shape):
+    """INT8 Ozaki form: within 1e-8 of the FP64 K and E_x, exactly symmetric,
+    and equal (to FP64 rounding of the fold, 1e-14) to the oracle's
+    restatement of the same slicing and exact level sums."""
+    from rest_tensors_amd._lib import lib
+    nbas, naux, nocc = shape
+    print(f"ozaki_int8 shape {shape}", flush=True)
+    dB = make_B(D, nbas, naux, nocc, seed=3000 + nbas)
+    K64, K8 = D.DeviceArray(nbas * nbas), D.DeviceArray(nbas * nbas)
+    ex64 = D.rik(dB, nocc, nbas, naux, K64, precision="fp64")
+    ex8 = D.rik(dB, nocc, nbas, naux, K8, precision="ozaki_int8")
+    k64, k8 = K64.to_host((nbas, nbas)), K8.to_host((nbas, nbas))
+    err = relf(k8, k64)
+    print(f"{shape}: K rel err ozaki_int8 {err:.2e}, E_x rel err "
+          f"{abs(ex8 - ex64) / abs(ex64):.2e}")
+    assert err < 1e-8 and abs(ex8 - ex64) < 1e-8 * abs(ex64)
+    assert np.array_equal(k8, k8.T)
+    if nocc * nbas * naux <= 4_000_000:
+        gp = lib().rt_ozaki_i8_group_size(nocc, nbas, naux)
+        B = dB.to_host((nocc, nbas, naux))
+        Ke = O.rik_ozaki_i8(B, gp)
+        assert relf(k8, Ke) < 1e-14
+        assert relf(k8, O.rik(B)[0]) < 1e-8

@@ -232,3 +232,66 @@ def test_special_dgemm_identity_and_permutation():
     Bm[1:6, 2:7] = perm
     out = O.special_dgemm_01(T, 0, 6, 4, 5, Bm, 1, 5, 2, 5, 1.0, 0.0)
     assert np.array_equal(out[:, :, 4:9], T[:, :, 4:9][:, :, [4, 0, 3, 1, 2]])
+
+
+# ---- int8 Ozaki slicing (the reduced-precision K form) ---------------------------
+def _ozaki_data(nocc=7, nmo=9, naux=11, seed=5):
+    rng = np.random.default_rng(seed)
+    B = rng.standard_normal((nocc, nmo, naux)) * np.exp(rng.uniform(-3, 3, (1, nmo, naux)))
+    B[:, 2, :] = 0.0   # an all-zero row (exponent 0, all slices 0)
+    return B
+
+
+def test_ozaki_i8_slices_reconstruct():
+    B = _ozaki_data()
+    for gp in (1, 3, 11):
+        sig, E = O.ozaki_i8_slices(B, gp)
+        assert np.all(sig == np.rint(sig)) and np.abs(sig).max() <= 127
+        w = np.array([254.0 ** -s for s in range(4)])
+        rec = np.einsum("s,smip->mip", w, sig) / 127.0
+        for g in range(E.shape[0]):
+            sl = slice(g * gp, (g + 1) * gp)
+            scale = np.ldexp(1.0, E[g])[None, :, None]
+            err = np.abs(B[:, :, sl] - rec[:, :, sl] * scale)
+            # truncation bound 2^E / 127 * 254^-3 / 2, plus FP64 rounding
+            assert np.all(err <= scale * (0.5 / 127.0 * 254.0 ** -3) * (1 + 1e-6))
+            mx = np.abs(B[:, :, sl]).max(axis=(0, 2))
+            assert np.all((mx == 0) | ((mx < np.ldexp(1.0, E[g])) & (mx >= np.ldexp(0.5, E[g]))))
+
+
+def test_ozaki_i8_k_brute_force():
+    """Level sums by explicit loops over (m, P) on a tiny case."""
+    B = _ozaki_data(nocc=3, nmo=4, naux=5, seed=7)
+    gp = 2
+    sig, E = O.ozaki_i8_slices(B, gp)
+    K = np.zeros((4, 4))
+    for g in range(E.shape[0]):
+        for i in range(4):
+            for j in range(4):
+                L = [0, 0, 0, 0]
+                for P in range(g * gp, min((g + 1) * gp, 5)):
+                    for m in range(3):
+                        for s in range(4):
+                            for t in range(4 - s):
+                                L[s + t] += int(sig[s, m, i, P]) * int(sig[t, m, j, P])
+                v = sum(L[q] * 254.0 ** -q for q in range(4)) / 127.0 ** 2
+                K[i, j] += v * 2.0 ** (E[g, i] + E[g, j])
+    Ko = O.rik_ozaki_i8(B, gp)
+    assert np.allclose(Ko, K, rtol=1e-14, atol=1e-14 * np.abs(K).max())
+
+
+def test_ozaki_i8_k_accuracy():
+    """Against the FP64 K: within 1e-8 relative Frobenius for several group
+    sizes; weights 256^-q in place of 254^-q would miss the bar."""
+    rng = np.random.default_rng(11)
+    B = rng.standard_normal((16, 40, 64))
+    K, _, _ = O.rik(B)
+    for gp in (1, 4, 64):
+        Ko = O.rik_ozaki_i8(B, gp)
+        assert np.linalg.norm(Ko - K) / np.linalg.norm(K) < 1e-8
+        assert np.array_equal(Ko, Ko.T)
+    sig, E = O.ozaki_i8_slices(B, 4)
+    bad = np.einsum("s,smip->mip", [256.0 ** -s for s in range(4)], sig) / 127.0
+    Bb = bad * np.ldexp(1.0, np.repeat(E, 4, axis=0).T)[None, :, :64]
+    Kb, _, _ = O.rik(Bb)
+    assert np.linalg.norm(Kb - K) / np.linalg.norm(K) > 1e-6
