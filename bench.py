@@ -56,6 +56,8 @@ def main():
                     help="molecules of the batched workload (0 = skip)")
     ap.add_argument("--davidson", type=int, default=1,
                     help="run the Davidson workload (plain vs learned preconditioner)")
+    ap.add_argument("--forms", default="ozaki_int8,ozaki_bf16,bf16x3",
+                    help="comma-separated reduced-precision K forms to time ('' = none)")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="only run one plain device copy of 512 MiB (copy3d kernel: reads "
                          "512 MiB, writes 512 MiB) to calibrate FETCH_SIZE / WRITE_SIZE")
@@ -138,8 +140,9 @@ def main():
     K64 = dK.to_host((nmo, nmo))
     prec = {}
     dK2 = D.DeviceArray(nmo * nmo)
-    for form, kname in (("ozaki_int8", "rik_oz_i8"), ("ozaki_bf16", "rik_ozaki"),
-                        ("bf16x3", "rik_bf16x3")):
+    knames = {"ozaki_int8": "rik_oz_i8", "ozaki_bf16": "rik_ozaki", "bf16x3": "rik_bf16x3"}
+    for form in [f for f in args.forms.split(",") if f]:
+        kname = knames[form]
         for _ in range(max(1, args.warmup)):
             D.rik(dB, nocc, nmo, naux, dK2, precision=form, want_energy=False)
         D.synchronize()
