@@ -198,9 +198,13 @@ typedef int (*rt_matvec_fn)(void* ctx, const double* X, double* Y, int64_t n,
 #define RT_PRECOND_DIAG_REF 0  /* r / (hdiag - e[0]) for every root (davidson.rs:224) */
 #define RT_PRECOND_DIAG_ROOT 1 /* r / (hdiag - e[k]) */
 #define RT_PRECOND_MLP 2       /* learned MLP correction (rt_mlp_*) */
+#define RT_PRECOND_AUX 3       /* the auxiliary operator's output (params.aux_op) */
 
 /* Optional recorder of (residual, Ritz vector, e_k, e_0, root, cycle) per
- * root per cycle, the training data of the learned preconditioner. */
+ * root per cycle, the training data of the learned preconditioner.  With
+ * xs != NULL it also keeps a copy of the subspace each cycle's corrections
+ * are orthogonalized against (while xs_cap allows): sample q's subspace is
+ * the xs_cnt[q] vectors at xs + xs_off[q] (xs_off[q] = -1: not kept). */
 typedef struct rt_davidson_record {
   double* r;     /* device, n * cap */
   double* x;     /* device, n * cap */
@@ -210,6 +214,11 @@ typedef struct rt_davidson_record {
   int* iter;     /* host, cap */
   int64_t cap;
   int64_t count; /* in/out: samples recorded so far */
+  double* xs;       /* device store of subspace copies, or NULL */
+  int64_t xs_cap;   /* doubles */
+  int64_t xs_used;  /* in/out: doubles used */
+  int64_t* xs_off;  /* host, cap */
+  int* xs_cnt;      /* host, cap */
 } rt_davidson_record;
 
 /* DavidsonParams (davidson.rs:28-46) plus the preconditioner choice. */
@@ -223,6 +232,11 @@ typedef struct rt_davidson_params {
   const void* mlp; /* network for RT_PRECOND_MLP */
   rt_davidson_record* record; /* may be NULL */
   int verbose;
+  /* auxiliary correction operator S = aux_op(aux_ctx, R) (e.g. a shift-invert
+   * of A): the correction itself for RT_PRECOND_AUX, an extra basis vector of
+   * the MLP for RT_PRECOND_MLP.  May be NULL. */
+  rt_matvec_fn aux_op;
+  void* aux_ctx;
 } rt_davidson_params;
 
 typedef struct rt_davidson_stats {
@@ -254,6 +268,17 @@ int rt_casida_op_destroy(void* op);
 int rt_casida_matvec(void* op, const double* X, double* Y, int64_t n,
                      int64_t k, void* stream); /* an rt_matvec_fn */
 int rt_casida_hdiag(void* op, double* hdiag, void* stream);
+/* Shift-invert of the operator, T = (A - sigma)^-1 R with sigma = min(d) -
+ * shift (shift > 0), by the Woodbury identity on the low-rank part; the
+ * naux x naux capacitance inverse is built once (Newton-Schulz on FP64 MFMA
+ * GEMMs).  rt_casida_si_apply is an rt_matvec_fn (aux_op of Davidson).
+ * Not in the reference: the structured correction of the learned
+ * preconditioner. */
+int rt_casida_si_create(void* op, double shift, void** si, void* stream);
+int rt_casida_si_destroy(void* si);
+int rt_casida_si_apply(void* si, const double* R, double* T, int64_t n,
+                       int64_t k, void* stream);
+int rt_casida_si_info(void* si, double* sigma, int* iters, double* err);
 
 /* ---- learned preconditioner: 2-layer FP32 MLP (hidden 64) ------------------ */
 int rt_mlp_num_params(void);
@@ -261,29 +286,38 @@ int rt_mlp_create(uint64_t seed, void** net);
 int rt_mlp_destroy(void* net);
 int rt_mlp_get_params(void* net, float* out);
 int rt_mlp_set_params(void* net, const float* in);
-/* T[:, s] = network correction for residual R[:, s], Ritz vector X[:, s] and
- * shifts ek[s], e0[s] (host arrays), s < ns. */
-int rt_mlp_apply_dev(void* net, const double* R, const double* X,
+/* Inputs of sample s (columns s of the device arrays, n rows): residual
+ * R[:, s], Ritz vector X[:, s], auxiliary correction S[:, s] (S may be NULL:
+ * that basis vector is then zero) and the shifts ek[s], e0[s] (host).
+ * T[:, s] = the network's correction. */
+int rt_mlp_apply_dev(void* net, const double* R, const double* X, const double* S,
                      const double* hdiag, const double* ek, const double* e0,
                      int64_t n, int ns, double* T, void* stream);
-/* loss (host scalar) and its gradient (host [rt_mlp_num_params()]) over ns
- * samples with device targets tau (n x ns) and host weights wts[ns]. */
+/* Loss (host scalar) and gradient (host [rt_mlp_num_params()]) over ns
+ * samples with device targets tau (n x ns) and host weights wts[ns].  When Q
+ * is given, sample s's correction is compared with tau after projecting out
+ * the q_cnt[s] orthonormal vectors at Q + q_off[s] (host arrays; q_off[s] < 0:
+ * no projection), as Davidson projects it before use. */
 int rt_mlp_loss_grad_dev(void* net, const double* R, const double* X,
-                         const double* hdiag, const double* ek,
+                         const double* S, const double* hdiag, const double* ek,
                          const double* e0, const double* tau, const double* wts,
-                         int64_t n, int ns, double* loss, double* grad,
+                         int64_t n, int ns, const double* Q, const int64_t* q_off,
+                         const int* q_cnt, double* loss, double* grad,
                          void* stream);
 /* Adam over `steps` minibatches of `batch` samples (seeded draw);
  * loss_hist: host [steps] or NULL. */
-int rt_mlp_train_dev(void* net, const double* R, const double* X,
+int rt_mlp_train_dev(void* net, const double* R, const double* X, const double* S,
                      const double* hdiag, const double* ek, const double* e0,
                      const double* tau, const double* wts, int64_t n, int ns,
+                     const double* Q, const int64_t* q_off, const int* q_cnt,
                      int steps, int batch, double lr, uint64_t seed,
                      double* loss_hist, void* stream);
-/* targets tau[:, s] = normalize(x* - (x*.x) x) with x* = Xfinal[:, root[s]]
- * (host root[]); wts[s] = 0 where |x*.x| < min_overlap (root mismatch). */
+/* targets tau[:, s] = normalize(x* - Q_s Q_s^T x*) with x* = Xfinal[:, root[s]]
+ * and Q_s sample s's subspace (without Q: normalize(x* - (x*.x) x));
+ * wts[s] = 0 where |x*.x| < min_overlap (root mismatch) or no subspace was kept. */
 int rt_mlp_targets_dev(const double* X, const int* root, int ns,
                        const double* Xfinal, int64_t n, double min_overlap,
+                       const double* Q, const int64_t* q_off, const int* q_cnt,
                        double* tau, double* wts, void* stream);
 
 /* ---- batched pipeline for many molecules (grouped launches + HIP graph) ---- */

@@ -75,17 +75,22 @@ RT_SIGNATURES = {
     "rt_casida_op_destroy": (i32, [vp]),
     "rt_casida_matvec": (i32, [vp, vp, vp, i64, i64, vp]),
     "rt_casida_hdiag": (i32, [vp, vp, vp]),
+    "rt_casida_si_create": (i32, [vp, f64, C.POINTER(vp), vp]),
+    "rt_casida_si_destroy": (i32, [vp]),
+    "rt_casida_si_apply": (i32, [vp, vp, vp, i64, i64, vp]),
+    "rt_casida_si_info": (i32, [vp, pf64, pi32, pf64]),
     "rt_mlp_num_params": (i32, []),
     "rt_mlp_create": (i32, [u64, C.POINTER(vp)]),
     "rt_mlp_destroy": (i32, [vp]),
     "rt_mlp_get_params": (i32, [vp, C.POINTER(C.c_float)]),
     "rt_mlp_set_params": (i32, [vp, C.POINTER(C.c_float)]),
-    "rt_mlp_apply_dev": (i32, [vp, vp, vp, vp, pf64, pf64, i64, i32, vp, vp]),
-    "rt_mlp_loss_grad_dev": (i32, [vp, vp, vp, vp, pf64, pf64, vp, pf64, i64, i32, pf64,
-                                   pf64, vp]),
-    "rt_mlp_train_dev": (i32, [vp, vp, vp, vp, pf64, pf64, vp, pf64, i64, i32, i32, i32,
-                               f64, u64, pf64, vp]),
-    "rt_mlp_targets_dev": (i32, [vp, pi32, i32, vp, i64, f64, vp, pf64, vp]),
+    "rt_mlp_apply_dev": (i32, [vp, vp, vp, vp, vp, pf64, pf64, i64, i32, vp, vp]),
+    "rt_mlp_loss_grad_dev": (i32, [vp, vp, vp, vp, vp, pf64, pf64, vp, pf64, i64, i32,
+                                   vp, pi64, pi32, pf64, pf64, vp]),
+    "rt_mlp_train_dev": (i32, [vp, vp, vp, vp, vp, pf64, pf64, vp, pf64, i64, i32,
+                               vp, pi64, pi32, i32, i32, f64, u64, pf64, vp]),
+    "rt_mlp_targets_dev": (i32, [vp, pi32, i32, vp, i64, f64, vp, pi64, pi32, vp, pf64,
+                                 vp]),
     # batched pipeline
     "rt_batch_create": (i32, [vp, i32, C.POINTER(vp)]),
     "rt_batch_autotune": (i32, [vp, i32, vp]),

@@ -122,6 +122,8 @@ extern "C" int rt_davidson_solve_dev(rt_matvec_fn a_x, void* ctx,
   RTA_CHECK(n - nroots >= 3, "Too much nroots required (n - nroots < 3)");
   RTA_CHECK(prm->precond != RT_PRECOND_MLP || prm->mlp,
             "MLP preconditioner requested without a network");
+  RTA_CHECK(prm->precond != RT_PRECOND_AUX || prm->aux_op,
+            "auxiliary preconditioner requested without an operator");
   hipStream_t s = (hipStream_t)stream;
   const double t_start = now_s();
   const double tol = prm->tol, lindep = prm->lindep;
@@ -129,13 +131,14 @@ extern "C" int rt_davidson_solve_dev(rt_matvec_fn a_x, void* ctx,
   const int cap = maxspace + 2 * nroots + 48;
   const int tmax = std::max(nroots, 40);
 
-  DevBuf XS, AX, X0, AX0, R, T, work, scr, small;
+  DevBuf XS, AX, X0, AX0, R, T, Sa, work, scr, small;
   RTA_TRY(XS.alloc((size_t)n * cap));
   RTA_TRY(AX.alloc((size_t)n * cap));
   RTA_TRY(X0.alloc((size_t)n * cap));
   RTA_TRY(AX0.alloc((size_t)n * cap));
   RTA_TRY(R.alloc((size_t)n * tmax));
   RTA_TRY(T.alloc((size_t)n * tmax));
+  if (prm->precond == RT_PRECOND_MLP && prm->aux_op) RTA_TRY(Sa.alloc((size_t)n * tmax));
   RTA_TRY(work.alloc((size_t)n));
   const int64_t scr_n =
       std::max<int64_t>(dv_gram_scratch(n, cap, tmax), dv_gram_scratch(n, 1, 1) * tmax);
@@ -253,8 +256,21 @@ extern "C" int rt_davidson_solve_dev(rt_matvec_fn a_x, void* ctx,
     for (int k = 0; k < nroots; ++k) all_conv = all_conv && conv[k];
     if (prm->record) {
       rt_davidson_record* rec = prm->record;
+      // the subspace this cycle's corrections are projected against
+      int64_t xs_off = -1;
+      if (rec->xs && rec->xs_off && rec->xs_cnt && rec->count < rec->cap &&
+          rec->xs_used + (int64_t)space * n <= rec->xs_cap) {
+        xs_off = rec->xs_used;
+        RTA_HIP(hipMemcpyAsync(rec->xs + xs_off, XS.p, sizeof(double) * n * space,
+                               hipMemcpyDeviceToDevice, s));
+        rec->xs_used += (int64_t)space * n;
+      }
       for (int k = 0; k < nroots_current && rec->count < rec->cap; ++k) {
         const int64_t q = rec->count++;
+        if (rec->xs_off && rec->xs_cnt) {
+          rec->xs_off[q] = xs_off;
+          rec->xs_cnt[q] = xs_off >= 0 ? space : 0;
+        }
         RTA_HIP(hipMemcpyAsync(rec->r + q * n, R.p + (size_t)k * n,
                                sizeof(double) * n, hipMemcpyDeviceToDevice, s));
         RTA_HIP(hipMemcpyAsync(rec->x + q * n, X0.p + (size_t)k * n,
@@ -291,9 +307,14 @@ extern "C" int rt_davidson_solve_dev(rt_matvec_fn a_x, void* ctx,
       }
     if (nt > 0) {
       if (prm->precond == RT_PRECOND_MLP) {
-        RTA_TRY(mlp_precondition((const MlpNet*)prm->mlp, R.p, X0.p, keep.data(), nt,
+        if (prm->aux_op)
+          RTA_TRY(prm->aux_op(prm->aux_ctx, R.p, Sa.p, n, nt, stream));
+        RTA_TRY(mlp_precondition((const MlpNet*)prm->mlp, R.p, X0.p,
+                                 prm->aux_op ? Sa.p : nullptr, keep.data(), nt,
                                  hdiag_dev, e.data(), n, T.p, scr.p, s,
                                  &st.mlp_fallbacks));
+      } else if (prm->precond == RT_PRECOND_AUX) {
+        RTA_TRY(prm->aux_op(prm->aux_ctx, R.p, T.p, n, nt, stream));
       } else {
         for (int j = 0; j < nt; ++j)
           h_shift[j] = (prm->precond == RT_PRECOND_DIAG_ROOT) ? e[keep[j]] : e[0];
@@ -387,4 +408,43 @@ extern "C" int rt_casida_matvec(void* op, const double* X, double* Y, int64_t n,
 extern "C" int rt_casida_hdiag(void* op, double* hdiag, void* stream) {
   RTA_CHECK(op && hdiag, "NULL argument");
   return casida_hdiag((const CasidaOp*)op, hdiag, (hipStream_t)stream);
+}
+
+extern "C" int rt_casida_si_create(void* op, double shift, void** si, void* stream) {
+  RTA_CHECK(op && si, "NULL argument");
+  CasidaSI* p = new CasidaSI{};
+  p->op = *(const CasidaOp*)op;
+  const int st = casida_si_setup(p, shift, (hipStream_t)stream);
+  if (st != RT_OK) {
+    if (p->Dm) (void)hipFree(p->Dm);
+    if (p->Ci) (void)hipFree(p->Ci);
+    delete p;
+    return st;
+  }
+  *si = p;
+  return RT_OK;
+}
+
+extern "C" int rt_casida_si_destroy(void* si) {
+  if (!si) return RT_OK;
+  CasidaSI* p = (CasidaSI*)si;
+  (void)hipFree(p->Dm);
+  (void)hipFree(p->Ci);
+  delete p;
+  return RT_OK;
+}
+
+extern "C" int rt_casida_si_apply(void* si, const double* R, double* T, int64_t n,
+                                  int64_t k, void* stream) {
+  RTA_CHECK(si && R && T, "NULL argument");
+  return casida_si_apply((const CasidaSI*)si, R, T, n, k, (hipStream_t)stream);
+}
+
+extern "C" int rt_casida_si_info(void* si, double* sigma, int* iters, double* err) {
+  RTA_CHECK(si, "NULL argument");
+  const CasidaSI* p = (const CasidaSI*)si;
+  if (sigma) *sigma = p->sigma;
+  if (iters) *iters = p->ns_iters;
+  if (err) *err = p->ns_err;
+  return RT_OK;
 }

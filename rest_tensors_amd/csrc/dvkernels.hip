@@ -6,6 +6,8 @@
 // All reductions run in a fixed order (deterministic results).
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 #include <algorithm>
 
 #include "common.hpp"
@@ -452,6 +454,188 @@ int casida_hdiag(const CasidaOp* op, double* hdiag, hipStream_t s) {
                      dim3(256), 0, s, op->B, op->ldb, op->naux, op->d,
                      2.0 * op->scale * op->scale, op->n, hdiag);
   RTA_HIP(hipGetLastError());
+  return RT_OK;
+}
+
+// ---- shift-invert preconditioner (Woodbury) -------------------------------------
+__global__ void si_dm_kernel(const double* __restrict__ d, double sigma, int64_t n,
+                             double* __restrict__ Dm) {
+  const int64_t x = blockIdx.x * (int64_t)256 + threadIdx.x;
+  if (x < n) Dm[x] = 1.0 / (d[x] - sigma);
+}
+
+// Bs[x + n*P] = sqrt(Dm[x]) B[x + ldb*P]
+__global__ void si_scale_rows_kernel(const double* __restrict__ B, int64_t ldb,
+                                     const double* __restrict__ Dm, int64_t n,
+                                     int64_t naux, double* __restrict__ Bs) {
+  const int64_t e = blockIdx.x * (int64_t)256 + threadIdx.x;
+  if (e >= n * naux) return;
+  const int64_t x = e % n, P = e / n;
+  Bs[e] = sqrt(Dm[x]) * B[x + ldb * P];
+}
+
+// M[i + m*i] += v
+__global__ void si_add_diag_kernel(double* __restrict__ M, int64_t m, double v) {
+  const int64_t i = blockIdx.x * (int64_t)256 + threadIdx.x;
+  if (i < m) M[i + m * i] += v;
+}
+
+// X = v * I (m x m), a fill kernel (no memset node in captured graphs)
+__global__ void si_scaled_eye_kernel(double* __restrict__ X, int64_t m, double v) {
+  const int64_t e = blockIdx.x * (int64_t)256 + threadIdx.x;
+  if (e >= m * m) return;
+  X[e] = (e % m == e / m) ? v : 0.0;
+}
+
+// E = I - T
+__global__ void si_eye_minus_kernel(const double* __restrict__ T, int64_t m,
+                                    double* __restrict__ E) {
+  const int64_t e = blockIdx.x * (int64_t)256 + threadIdx.x;
+  if (e >= m * m) return;
+  E[e] = ((e % m == e / m) ? 1.0 : 0.0) - T[e];
+}
+
+__global__ void si_copy_kernel(const double* __restrict__ a, int64_t len,
+                               double* __restrict__ b) {
+  const int64_t e = blockIdx.x * (int64_t)256 + threadIdx.x;
+  if (e < len) b[e] = a[e];
+}
+
+// G = Dm o R (n x k)
+__global__ void si_diag_apply_kernel(const double* __restrict__ Dm,
+                                     const double* __restrict__ R, int64_t n,
+                                     int64_t k, double* __restrict__ G) {
+  const int64_t e = blockIdx.x * (int64_t)256 + threadIdx.x;
+  if (e < n * k) G[e] = Dm[e % n] * R[e];
+}
+
+// T = G - Dm o sum_i Ypart[i]
+__global__ void si_finish_kernel(const double* __restrict__ Ypart, int nslice,
+                                 const double* __restrict__ Dm,
+                                 const double* __restrict__ G, int64_t n, int k,
+                                 double* __restrict__ T) {
+  const int64_t e = blockIdx.x * (int64_t)256 + threadIdx.x;
+  if (e >= n * k) return;
+  double acc = 0.0;
+  for (int i = 0; i < nslice; ++i) acc += Ypart[(int64_t)i * n * k + e];
+  T[e] = G[e] - Dm[e % n] * acc;
+}
+
+static unsigned si_grid(int64_t len) { return (unsigned)((len + 255) / 256); }
+
+int casida_si_setup(CasidaSI* si, double shift, hipStream_t s) {
+  const CasidaOp& op = si->op;
+  const int64_t n = op.n, m = op.naux;
+  RTA_CHECK(shift > 0.0, "shift-invert shift must be positive");
+  RTA_CHECK(op.scale != 0.0, "operator without a low-rank part");
+  std::vector<double> dh(n);
+  RTA_HIP(hipMemcpyAsync(dh.data(), op.d, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+  RTA_HIP(hipStreamSynchronize(s));
+  double dmin = dh[0];
+  for (int64_t x = 1; x < n; ++x) dmin = std::min(dmin, dh[x]);
+  si->sigma = dmin - shift;
+  const double c = 2.0 * op.scale * op.scale;
+  RTA_HIP(hipMalloc(&si->Dm, sizeof(double) * n));
+  RTA_HIP(hipMalloc(&si->Ci, sizeof(double) * m * m));
+  hipLaunchKernelGGL(si_dm_kernel, dim3(si_grid(n)), dim3(256), 0, s, op.d, si->sigma,
+                     n, si->Dm);
+  double *Bs = nullptr, *Cap = nullptr, *Xn = nullptr, *T = nullptr, *sq = nullptr;
+  int st = RT_OK;
+  auto alloc = [&](double** p, size_t cnt) {
+    if (hipMalloc(p, sizeof(double) * cnt) != hipSuccess) {
+      (void)hipGetLastError();
+      *p = nullptr;
+      return false;
+    }
+    return true;
+  };
+  if (!alloc(&Bs, (size_t)n * m) || !alloc(&Cap, (size_t)m * m) ||
+      !alloc(&Xn, (size_t)m * m) || !alloc(&T, (size_t)m * m) ||
+      !alloc(&sq, 64 * 1024)) {
+    st = fail(RT_ERR_NOMEM, "shift-invert setup: allocation failed");
+  }
+  double* X = si->Ci;
+  double err = 0.0;
+  int it = 0;
+  if (st == RT_OK) {
+    // Cap = I/c + Bs^T Bs, Bs = sqrt(Dm) o B
+    hipLaunchKernelGGL(si_scale_rows_kernel, dim3(si_grid(n * m)), dim3(256), 0, s, op.B,
+                       op.ldb, si->Dm, n, m, Bs);
+    st = rt_dgemm_dev('T', 'N', m, m, n, 1.0, Bs, n, Bs, n, 0.0, Cap, m, s);
+  }
+  if (st == RT_OK) {
+    hipLaunchKernelGGL(si_add_diag_kernel, dim3(si_grid(m)), dim3(256), 0, s, Cap, m,
+                       1.0 / c);
+    // Newton-Schulz: X0 = I/|Cap|_F, X <- 2X - X Cap X; E = I - Cap X squares each step
+    st = dv_sqnorms(Cap, 1, m * m, sq, sq + 1, s);
+  }
+  if (st == RT_OK) {
+    double fro2 = 0.0;
+    RTA_HIP(hipMemcpyAsync(&fro2, sq, sizeof(double), hipMemcpyDeviceToHost, s));
+    RTA_HIP(hipStreamSynchronize(s));
+    hipLaunchKernelGGL(si_scaled_eye_kernel, dim3(si_grid(m * m)), dim3(256), 0, s, X, m,
+                       1.0 / std::sqrt(fro2));
+    for (it = 0; it < 60 && st == RT_OK; ++it) {
+      // Cap and X are symmetric: T = Cap^T X, X_new = 2X - X^T T
+      st = rt_dgemm_dev('T', 'N', m, m, m, 1.0, Cap, m, X, m, 0.0, T, m, s);
+      if (st != RT_OK) break;
+      hipLaunchKernelGGL(si_eye_minus_kernel, dim3(si_grid(m * m)), dim3(256), 0, s, T, m,
+                         Xn);
+      st = dv_sqnorms(Xn, 1, m * m, sq, sq + 1, s);
+      if (st != RT_OK) break;
+      RTA_HIP(hipMemcpyAsync(&err, sq, sizeof(double), hipMemcpyDeviceToHost, s));
+      RTA_HIP(hipStreamSynchronize(s));
+      err = std::sqrt(err);
+      if (err < 1e-10) break;
+      hipLaunchKernelGGL(si_copy_kernel, dim3(si_grid(m * m)), dim3(256), 0, s, X, m * m,
+                         Xn);
+      st = rt_dgemm_dev('T', 'N', m, m, m, -1.0, X, m, T, m, 2.0, Xn, m, s);
+      if (st != RT_OK) break;
+      hipLaunchKernelGGL(si_copy_kernel, dim3(si_grid(m * m)), dim3(256), 0, s, Xn, m * m,
+                         X);
+    }
+    if (st == RT_OK && !(err < 1e-6))
+      st = fail(RT_ERR_NUMERIC, "shift-invert: Newton-Schulz stalled at |I - C X| = %.3e",
+                err);
+  }
+  (void)hipStreamSynchronize(s);
+  if (Bs) (void)hipFree(Bs);
+  if (Cap) (void)hipFree(Cap);
+  if (Xn) (void)hipFree(Xn);
+  if (T) (void)hipFree(T);
+  if (sq) (void)hipFree(sq);
+  si->ns_iters = it;
+  si->ns_err = err;
+  return st;
+}
+
+int casida_si_apply(const CasidaSI* si, const double* R, double* T, int64_t n,
+                    int64_t k, hipStream_t s) {
+  const CasidaOp& op = si->op;
+  RTA_CHECK(n == op.n, "vector length %lld != operator dimension %lld", (long long)n,
+            (long long)op.n);
+  if (k <= 0) return RT_OK;
+  const int64_t m = op.naux;
+  const int nslice = 8;
+  const int64_t pslice = (m + nslice - 1) / nslice;
+  double *G, *Z, *Ypart;
+  RTA_TRY(scratch_get(SCRATCH_DV_Z, sizeof(double) * (2 * m * 8 + n * 8), (void**)&Z));
+  double* W = Z + m * 8;
+  G = W + m * 8;
+  RTA_TRY(scratch_get(SCRATCH_DV_YPART, sizeof(double) * nslice * n * 8, (void**)&Ypart));
+  for (int64_t c0 = 0; c0 < k; c0 += 8) {
+    const int kc = (int)std::min<int64_t>(8, k - c0);
+    hipLaunchKernelGGL(si_diag_apply_kernel, dim3(si_grid(n * kc)), dim3(256), 0, s,
+                       si->Dm, R + c0 * n, n, (int64_t)kc, G);
+    hipLaunchKernelGGL(skinny_bt_x_kernel<8>, dim3((unsigned)((m + 7) / 8)), dim3(256), 0,
+                       s, op.B, op.ldb, m, G, n, kc, Z);
+    RTA_TRY(rt_dgemm_dev('T', 'N', m, kc, m, 1.0, si->Ci, m, Z, m, 0.0, W, m, s));
+    hipLaunchKernelGGL(skinny_b_z_kernel<8>, dim3((unsigned)((n + 255) / 256), nslice),
+                       dim3(256), 0, s, op.B, op.ldb, m, W, n, kc, pslice, Ypart);
+    hipLaunchKernelGGL(si_finish_kernel, dim3(si_grid(n * kc)), dim3(256), 0, s, Ypart,
+                       nslice, si->Dm, G, n, kc, T + c0 * n);
+    RTA_HIP(hipGetLastError());
+  }
   return RT_OK;
 }
 

@@ -21,6 +21,25 @@ int casida_apply(const CasidaOp* op, const double* X, double* Y, int64_t n,
                  int64_t k, hipStream_t s);
 int casida_hdiag(const CasidaOp* op, double* hdiag, hipStream_t s);
 
+// Shift-invert preconditioner of the Casida operator: T = (A - sigma)^-1 R by
+// the Woodbury identity with A = D + c B B^T, c = 2 scale^2, Dm = 1/(d - sigma)
+// (sigma below min d):
+//   (A - sigma)^-1 = Dm - Dm B (I/c + B^T Dm B)^-1 B^T Dm.
+// Ci = (I/c + B^T Dm B)^-1 (naux x naux) is formed once, by Newton-Schulz
+// iteration on FP64 GEMMs.
+struct CasidaSI {
+  CasidaOp op;
+  double sigma;
+  double* Dm;   // device, n
+  double* Ci;   // device, naux x naux
+  int ns_iters; // Newton-Schulz iterations used
+  double ns_err; // |I - Cap Ci|_F at the end
+};
+
+int casida_si_setup(CasidaSI* si, double shift, hipStream_t s);
+int casida_si_apply(const CasidaSI* si, const double* R, double* T, int64_t n,
+                    int64_t k, hipStream_t s);
+
 // G[i + ldg*j] = U[:,i] . W[:,j], i < nu, j < nw; scratch >= dv_gram_scratch
 int dv_gram(const double* U, int64_t ldu, int nu, const double* W, int64_t ldw,
             int nw, int64_t n, double* G, int64_t ldg, double* scratch,

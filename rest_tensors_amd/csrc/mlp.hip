@@ -2,12 +2,17 @@
 // training kernels of the 2-layer FP32 MLP described in mlp.hpp.
 //
 // Training signal: samples recorded from plain Davidson runs (residual r and
-// Ritz vector x0 of root k at some cycle) paired with that run's converged
-// eigenvector x*; the target direction is tau = x* - (x*.x0) x0 (the part of
-// the answer the current estimate lacks), normalized.  The loss is
-//   L = sum_s w_s (1 - (t_s.tau_s)^2 / (|t_s|^2 |tau_s|^2)) / sum_s w_s,
-// the squared-cosine misalignment of the network's correction t_s with tau_s
-// (sign-free, scale-free, as Davidson only uses the direction of t).
+// Ritz vector x0 of root k at some cycle, and the subspace Q that cycle's
+// corrections are orthogonalized against) paired with that run's converged
+// eigenvector x*; the target direction is tau = x* - Q Q^T x* (the part of
+// the answer the subspace lacks), normalized.  With p_s = (I - Q Q^T) t_s,
+// the correction as Davidson uses it, the loss is
+//   L = sum_s w_s (1 - (p_s.tau_s)^2 / (|p_s|^2 |tau_s|^2)) / sum_s w_s,
+// the squared-cosine misalignment of the projected correction with tau_s
+// (sign-free, scale-free, as Davidson only uses the direction of p).  Its
+// gradient with respect to t_s is (I - Q Q^T) applied to the gradient with
+// respect to p_s, which equals -2a tau/(b c) + 2a^2 p/(b^2 c) since tau and p
+// already lie in the complement (a = p.tau, b = p.p, c = tau.tau).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -15,6 +20,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "dvkernels.hpp"
 #include "mlp.hpp"
 #include "rng.hpp"
 #include "timer.hpp"
@@ -24,13 +30,15 @@ namespace rta {
 struct MlpSample {
   const double* R;      // residuals, column s at R + s*n
   const double* X;      // Ritz vectors, column xcol[s] at X + xcol[s]*n
+  const double* A;      // auxiliary corrections, column s at A + s*n (or NULL)
   const int* xcol;      // device
   const double* ek;     // device, per sample
   const double* e0;     // device, per sample
-  const double* stats;  // device, 3 per sample: sum r^2, sum gk^2, sum g0^2
+  const double* stats;  // device, 4 per sample: sum r^2, gk^2, g0^2, a^2
   const double* hdiag;
   int64_t n;
 };
+constexpr int MLP_NSTAT = 4;
 
 __device__ __forceinline__ double mlp_safe(double h) {
   return fabs(h) < 1e-8 ? 1e-8 : h;
@@ -42,27 +50,28 @@ __device__ __forceinline__ double mlp_wsum(double v) {
   return v;
 }
 
-// basis b[4] and features f[6] of component x of sample s
+// basis b[5] and features f[7] of component x of sample s
 __device__ __forceinline__ void mlp_inputs(const MlpSample& S, int s, int64_t x,
                                            double b[MLP_O], float f[MLP_F]) {
   const double r = S.R[x + S.n * s];
   const double x0 = S.X[x + S.n * S.xcol[s]];
+  const double a = S.A ? S.A[x + S.n * s] : 0.0;
   const double hd = S.hdiag[x];
   const double hk = hd - S.ek[s], h0 = hd - S.e0[s];
   const double gk = r / mlp_safe(hk), g0 = r / mlp_safe(h0);
   const double nn = (double)S.n;
-  const double* st = S.stats + 3 * s;
-  const double rr = sqrt(st[0] / nn), rk = sqrt(st[1] / nn), r0 = sqrt(st[2] / nn);
+  const double* st = S.stats + MLP_NSTAT * s;
+  const double rr = sqrt(st[0] / nn), rk = sqrt(st[1] / nn), r0 = sqrt(st[2] / nn),
+               ra = sqrt(st[3] / nn);
   b[0] = rk > 0 ? gk / rk : 0.0;
   b[1] = r0 > 0 ? g0 / r0 : 0.0;
   b[2] = x0 * sqrt(nn);
   b[3] = rr > 0 ? r / rr : 0.0;
-  f[0] = asinhf((float)b[0]);
-  f[1] = asinhf((float)b[1]);
-  f[2] = asinhf((float)b[2]);
-  f[3] = asinhf((float)b[3]);
-  f[4] = tanhf((float)hk / MLP_SHIFT_SCALE);
-  f[5] = tanhf((float)h0 / MLP_SHIFT_SCALE);
+  b[4] = ra > 0 ? a / ra : 0.0;
+#pragma unroll
+  for (int q = 0; q < 5; ++q) f[q] = asinhf((float)b[q]);
+  f[5] = tanhf((float)hk / MLP_SHIFT_SCALE);
+  f[6] = tanhf((float)h0 / MLP_SHIFT_SCALE);
 }
 
 // hidden activations z and mixing weights w from features (params in LDS)
@@ -82,36 +91,35 @@ __device__ __forceinline__ void mlp_core(const float* __restrict__ P,
   }
 }
 
-// stats: sum r^2, sum gk^2, sum g0^2 per sample (partials over x slices)
+// stats: sum r^2, gk^2, g0^2, a^2 per sample (partials over x slices)
 __global__ __launch_bounds__(256) void mlp_stats_kernel(MlpSample S,
                                                         int64_t slice,
                                                         double* __restrict__ part) {
-  __shared__ double red[3][4];
+  __shared__ double red[MLP_NSTAT][4];
   const int s = blockIdx.y;
   const int64_t x0 = (int64_t)blockIdx.x * slice;
   const int64_t x1 = (x0 + slice < S.n) ? x0 + slice : S.n;
-  double a0 = 0, a1 = 0, a2 = 0;
+  double acc[MLP_NSTAT] = {0, 0, 0, 0};
   for (int64_t x = x0 + threadIdx.x; x < x1; x += 256) {
     const double r = S.R[x + S.n * s];
     const double hd = S.hdiag[x];
     const double gk = r / mlp_safe(hd - S.ek[s]), g0 = r / mlp_safe(hd - S.e0[s]);
-    a0 += r * r;
-    a1 += gk * gk;
-    a2 += g0 * g0;
+    const double a = S.A ? S.A[x + S.n * s] : 0.0;
+    acc[0] += r * r;
+    acc[1] += gk * gk;
+    acc[2] += g0 * g0;
+    acc[3] += a * a;
   }
-  a0 = mlp_wsum(a0);
-  a1 = mlp_wsum(a1);
-  a2 = mlp_wsum(a2);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  if (lane == 0) {
-    red[0][wave] = a0;
-    red[1][wave] = a1;
-    red[2][wave] = a2;
+#pragma unroll
+  for (int q = 0; q < MLP_NSTAT; ++q) {
+    const double v = mlp_wsum(acc[q]);
+    if (lane == 0) red[q][wave] = v;
   }
   __syncthreads();
-  if (threadIdx.x < 3) {
+  if (threadIdx.x < MLP_NSTAT) {
     const int q = threadIdx.x;
-    part[((int64_t)s * gridDim.x + blockIdx.x) * 3 + q] =
+    part[((int64_t)s * gridDim.x + blockIdx.x) * MLP_NSTAT + q] =
         (red[q][0] + red[q][1]) + (red[q][2] + red[q][3]);
   }
 }
@@ -119,10 +127,11 @@ __global__ __launch_bounds__(256) void mlp_stats_kernel(MlpSample S,
 __global__ void mlp_stats_reduce(const double* __restrict__ part, int nsplit,
                                  int ns, double* __restrict__ stats) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= ns * 3) return;
-  const int s = e / 3, q = e % 3;
+  if (e >= ns * MLP_NSTAT) return;
+  const int s = e / MLP_NSTAT, q = e % MLP_NSTAT;
   double acc = 0.0;
-  for (int i = 0; i < nsplit; ++i) acc += part[((int64_t)s * nsplit + i) * 3 + q];
+  for (int i = 0; i < nsplit; ++i)
+    acc += part[((int64_t)s * nsplit + i) * MLP_NSTAT + q];
   stats[e] = acc;
 }
 
@@ -177,12 +186,13 @@ __global__ __launch_bounds__(256) void mlp_dots_kernel(
 }
 
 // Per-block partial gradients.  coef[s*2 + {0,1}] = (-2 a w/(W b c), 2 a^2 w/(W b^2 c))
-// so that dL/dt = coef0 * tau + coef1 * t.  128 components per block.
+// so that dL/dt = coef0 * tau + coef1 * p, p = TP (the projected correction)
+// or, without TP, the network output t.  128 components per block.
 constexpr int MLP_BX = 128;
 __global__ __launch_bounds__(MLP_BX) void mlp_backward_kernel(
     MlpSample S, const float* __restrict__ params,
     const double* __restrict__ tau, const double* __restrict__ coef,
-    float* __restrict__ gpart) {
+    const double* __restrict__ TP, float* __restrict__ gpart) {
   __shared__ float P[MLP_NPARAM];
   __shared__ float sf[MLP_BX][MLP_F];
   __shared__ float sz[MLP_BX][MLP_H + 1];
@@ -201,6 +211,7 @@ __global__ __launch_bounds__(MLP_BX) void mlp_backward_kernel(
     double t = 0.0;
 #pragma unroll
     for (int q = 0; q < MLP_O; ++q) t += (double)w[q] * b[q];
+    if (TP) t = TP[x + S.n * s];  // the projected correction
     const double gt = coef[2 * s] * tau[x + S.n * s] + coef[2 * s + 1] * t;
     float dw[MLP_O];
 #pragma unroll
@@ -319,8 +330,8 @@ static int nsplit_for(int64_t n) {
   return (int)std::max<int64_t>(1, std::min<int64_t>(64, (n + 4095) / 4096));
 }
 
-// Device workspace for ns samples: xcol (ns ints), ek, e0 (ns), stats (3 ns),
-// partials.  Lives in scratch slots (grow on demand).
+// Device workspace for ns samples: xcol (ns ints), ek, e0 (ns), stats
+// (MLP_NSTAT ns), partials.  Lives in scratch slots (grow on demand).
 struct MlpWork {
   int* xcol;
   double* ek;
@@ -331,8 +342,8 @@ struct MlpWork {
 
 static int mlp_work(int ns, int64_t n, MlpWork* w) {
   const int nsp = nsplit_for(n);
-  const size_t bytes = sizeof(int) * (ns + 2) + sizeof(double) * (2 * ns + 3 * ns) +
-                       sizeof(double) * 3 * (size_t)ns * nsp + 256;
+  const size_t bytes = sizeof(int) * (ns + 2) + sizeof(double) * (2 * ns + MLP_NSTAT * ns) +
+                       sizeof(double) * MLP_NSTAT * (size_t)ns * nsp + 256;
   char* p = nullptr;
   RTA_TRY(scratch_get(SCRATCH_MLP_A, bytes, (void**)&p));
   size_t off = 0;
@@ -344,31 +355,31 @@ static int mlp_work(int ns, int64_t n, MlpWork* w) {
   w->xcol = (int*)take(sizeof(int) * ns);
   w->ek = (double*)take(sizeof(double) * ns);
   w->e0 = (double*)take(sizeof(double) * ns);
-  w->stats = (double*)take(sizeof(double) * 3 * ns);
-  w->part = (double*)take(sizeof(double) * 3 * (size_t)ns * nsp);
+  w->stats = (double*)take(sizeof(double) * MLP_NSTAT * ns);
+  w->part = (double*)take(sizeof(double) * MLP_NSTAT * (size_t)ns * nsp);
   return RT_OK;
 }
 
 static int mlp_prepare(const MlpWork& w, MlpSample* S, const double* R,
-                       const double* X, const double* hdiag, int64_t n, int ns,
-                       const int* xcol_h, const double* ek_h, const double* e0_h,
-                       hipStream_t s) {
+                       const double* X, const double* A, const double* hdiag,
+                       int64_t n, int ns, const int* xcol_h, const double* ek_h,
+                       const double* e0_h, hipStream_t s) {
   RTA_HIP(hipMemcpyAsync(w.xcol, xcol_h, sizeof(int) * ns, hipMemcpyHostToDevice, s));
   RTA_HIP(hipMemcpyAsync(w.ek, ek_h, sizeof(double) * ns, hipMemcpyHostToDevice, s));
   RTA_HIP(hipMemcpyAsync(w.e0, e0_h, sizeof(double) * ns, hipMemcpyHostToDevice, s));
-  *S = MlpSample{R, X, w.xcol, w.ek, w.e0, w.stats, hdiag, n};
+  *S = MlpSample{R, X, A, w.xcol, w.ek, w.e0, w.stats, hdiag, n};
   const int nsp = nsplit_for(n);
   const int64_t slice = (n + nsp - 1) / nsp;
   hipLaunchKernelGGL(mlp_stats_kernel, dim3(nsp, ns), dim3(256), 0, s, *S, slice,
                      w.part);
-  hipLaunchKernelGGL(mlp_stats_reduce, dim3((3 * ns + 255) / 256), dim3(256), 0, s,
-                     w.part, nsp, ns, w.stats);
+  hipLaunchKernelGGL(mlp_stats_reduce, dim3((MLP_NSTAT * ns + 255) / 256), dim3(256), 0,
+                     s, w.part, nsp, ns, w.stats);
   RTA_HIP(hipGetLastError());
   return RT_OK;
 }
 
 int mlp_precondition(const MlpNet* net, const double* R, const double* X0,
-                     const int* keep, int nt, const double* hdiag,
+                     const double* Saux, const int* keep, int nt, const double* hdiag,
                      const double* e_host, int64_t n, double* T,
                      double* /*scratch*/, hipStream_t s, int* /*fallbacks*/) {
   if (nt <= 0) return RT_OK;
@@ -377,7 +388,7 @@ int mlp_precondition(const MlpNet* net, const double* R, const double* X0,
   std::vector<double> ek(nt), e0(nt, e_host[0]);
   for (int j = 0; j < nt; ++j) ek[j] = e_host[keep[j]];
   MlpSample S;
-  RTA_TRY(mlp_prepare(w, &S, R, X0, hdiag, n, nt, keep, ek.data(), e0.data(), s));
+  RTA_TRY(mlp_prepare(w, &S, R, X0, Saux, hdiag, n, nt, keep, ek.data(), e0.data(), s));
   KernelTimer kt("mlp_forward", s);
   hipLaunchKernelGGL(mlp_forward_kernel, dim3((unsigned)((n + 255) / 256), nt),
                      dim3(256), 0, s, S, net->params, T);
@@ -387,20 +398,28 @@ int mlp_precondition(const MlpNet* net, const double* R, const double* X0,
   return RT_OK;
 }
 
-// Loss and gradient over samples [0, ns) (columns of R/X/tau), FP64 result.
+// Loss and gradient over samples [0, ns) (columns of R/X/A/tau), FP64
+// result.  Qs[j], qc[j]: sample j's subspace (NULL / 0: no projection).
 static int mlp_loss_grad(const MlpNet* net, const double* R, const double* X,
-                         const double* hdiag, const double* ek_h,
-                         const double* e0_h, const double* tau,
-                         const double* wts_h, int64_t n, int ns, double* loss,
-                         double* grad_dev, double* T, hipStream_t s) {
+                         const double* A, const double* hdiag, const double* ek_h,
+                         const double* e0_h, const double* tau, const double* wts_h,
+                         int64_t n, int ns, const double* const* Qs, const int* qc,
+                         double* loss, double* grad_dev, double* T, hipStream_t s) {
   MlpWork w;
   RTA_TRY(mlp_work(ns, n, &w));
   std::vector<int> xcol(ns);
   for (int j = 0; j < ns; ++j) xcol[j] = j;
   MlpSample S;
-  RTA_TRY(mlp_prepare(w, &S, R, X, hdiag, n, ns, xcol.data(), ek_h, e0_h, s));
+  RTA_TRY(mlp_prepare(w, &S, R, X, A, hdiag, n, ns, xcol.data(), ek_h, e0_h, s));
   hipLaunchKernelGGL(mlp_forward_kernel, dim3((unsigned)((n + 255) / 256), ns),
                      dim3(256), 0, s, S, net->params, T);
+  bool projected = false;
+  if (Qs)
+    for (int j = 0; j < ns; ++j)
+      if (Qs[j] && qc[j] > 0) {
+        RTA_TRY(dv_project_out(Qs[j], n, qc[j], T + (size_t)j * n, 1, n, s));
+        projected = true;
+      }
   const int nsp = nsplit_for(n);
   const int64_t slice = (n + nsp - 1) / nsp;
   hipLaunchKernelGGL(mlp_dots_kernel, dim3(nsp, ns), dim3(256), 0, s, T, tau, n,
@@ -438,12 +457,25 @@ static int mlp_loss_grad(const MlpNet* net, const double* R, const double* X,
   RTA_HIP(hipMemcpyAsync(dcoef, coef.data(), sizeof(double) * 2 * ns,
                          hipMemcpyHostToDevice, s));
   hipLaunchKernelGGL(mlp_backward_kernel, dim3((unsigned)nbx, ns), dim3(MLP_BX), 0, s,
-                     S, net->params, tau, dcoef, gpart);
+                     S, net->params, tau, dcoef, projected ? T : nullptr, gpart);
   hipLaunchKernelGGL(mlp_grad_reduce, dim3((MLP_NPARAM + 255) / 256), dim3(256), 0,
                      s, gpart, nbx * ns, grad_dev);
   RTA_HIP(hipGetLastError());
   RTA_HIP(hipStreamSynchronize(s));  // host-side coef must outlive the copy
   return RT_OK;
+}
+
+// per-sample subspace pointers from (Q, q_off, q_cnt)
+static void mlp_subspaces(const double* Q, const int64_t* q_off, const int* q_cnt,
+                          int ns, std::vector<const double*>& Qs, std::vector<int>& qc) {
+  Qs.assign(ns, nullptr);
+  qc.assign(ns, 0);
+  if (!Q || !q_off || !q_cnt) return;
+  for (int j = 0; j < ns; ++j)
+    if (q_off[j] >= 0 && q_cnt[j] > 0) {
+      Qs[j] = Q + q_off[j];
+      qc[j] = q_cnt[j];
+    }
 }
 
 }  // namespace rta
@@ -506,7 +538,7 @@ extern "C" int rt_mlp_set_params(void* net, const float* in) {
 }
 
 extern "C" int rt_mlp_apply_dev(void* net, const double* R, const double* X,
-                                const double* hdiag, const double* ek,
+                                const double* A, const double* hdiag, const double* ek,
                                 const double* e0, int64_t n, int ns, double* T,
                                 void* stream) {
   RTA_CHECK(net && R && X && hdiag && ek && e0 && T, "NULL argument");
@@ -517,7 +549,7 @@ extern "C" int rt_mlp_apply_dev(void* net, const double* R, const double* X,
   std::vector<int> xcol(ns);
   for (int j = 0; j < ns; ++j) xcol[j] = j;
   MlpSample S;
-  RTA_TRY(mlp_prepare(w, &S, R, X, hdiag, n, ns, xcol.data(), ek, e0, s));
+  RTA_TRY(mlp_prepare(w, &S, R, X, A, hdiag, n, ns, xcol.data(), ek, e0, s));
   hipLaunchKernelGGL(mlp_forward_kernel, dim3((unsigned)((n + 255) / 256), ns),
                      dim3(256), 0, s, S, ((MlpNet*)net)->params, T);
   RTA_HIP(hipGetLastError());
@@ -526,19 +558,24 @@ extern "C" int rt_mlp_apply_dev(void* net, const double* R, const double* X,
 }
 
 extern "C" int rt_mlp_loss_grad_dev(void* net, const double* R, const double* X,
-                                    const double* hdiag, const double* ek,
-                                    const double* e0, const double* tau,
-                                    const double* wts, int64_t n, int ns,
-                                    double* loss, double* grad, void* stream) {
+                                    const double* A, const double* hdiag,
+                                    const double* ek, const double* e0,
+                                    const double* tau, const double* wts, int64_t n,
+                                    int ns, const double* Q, const int64_t* q_off,
+                                    const int* q_cnt, double* loss, double* grad,
+                                    void* stream) {
   RTA_CHECK(net && R && X && hdiag && ek && e0 && tau && wts && loss && grad,
             "NULL argument");
   RTA_CHECK(n > 0 && ns > 0, "empty input");
   hipStream_t s = (hipStream_t)stream;
+  std::vector<const double*> Qs;
+  std::vector<int> qc;
+  mlp_subspaces(Q, q_off, q_cnt, ns, Qs, qc);
   double *T = nullptr, *g = nullptr;
   RTA_HIP(hipMalloc(&T, sizeof(double) * n * ns));
   RTA_HIP(hipMalloc(&g, sizeof(double) * MLP_NPARAM));
-  int st = mlp_loss_grad((MlpNet*)net, R, X, hdiag, ek, e0, tau, wts, n, ns, loss,
-                         g, T, s);
+  int st = mlp_loss_grad((MlpNet*)net, R, X, A, hdiag, ek, e0, tau, wts, n, ns,
+                         Qs.data(), qc.data(), loss, g, T, s);
   if (st == RT_OK) {
     hipError_t e = hipMemcpy(grad, g, sizeof(double) * MLP_NPARAM,
                              hipMemcpyDeviceToHost);
@@ -550,19 +587,24 @@ extern "C" int rt_mlp_loss_grad_dev(void* net, const double* R, const double* X,
 }
 
 extern "C" int rt_mlp_train_dev(void* net_, const double* R, const double* X,
-                                const double* hdiag, const double* ek,
-                                const double* e0, const double* tau,
-                                const double* wts, int64_t n, int ns, int steps,
-                                int batch, double lr, uint64_t seed,
-                                double* loss_hist, void* stream) {
+                                const double* A, const double* hdiag,
+                                const double* ek, const double* e0,
+                                const double* tau, const double* wts, int64_t n,
+                                int ns, const double* Q, const int64_t* q_off,
+                                const int* q_cnt, int steps, int batch, double lr,
+                                uint64_t seed, double* loss_hist, void* stream) {
   RTA_CHECK(net_ && R && X && hdiag && ek && e0 && tau && wts, "NULL argument");
   RTA_CHECK(n > 0 && ns > 0 && steps >= 0 && batch > 0, "bad training sizes");
   MlpNet* net = (MlpNet*)net_;
   hipStream_t s = (hipStream_t)stream;
   batch = std::min(batch, ns);
-  double *Rb, *Xb, *Tb, *taub, *g;
+  std::vector<const double*> Qs_all, Qs(batch);
+  std::vector<int> qc_all, qc(batch);
+  mlp_subspaces(Q, q_off, q_cnt, ns, Qs_all, qc_all);
+  double *Rb, *Xb, *Ab = nullptr, *Tb, *taub, *g;
   RTA_HIP(hipMalloc(&Rb, sizeof(double) * n * batch));
   RTA_HIP(hipMalloc(&Xb, sizeof(double) * n * batch));
+  if (A) RTA_HIP(hipMalloc(&Ab, sizeof(double) * n * batch));
   RTA_HIP(hipMalloc(&Tb, sizeof(double) * n * batch));
   RTA_HIP(hipMalloc(&taub, sizeof(double) * n * batch));
   RTA_HIP(hipMalloc(&g, sizeof(double) * MLP_NPARAM));
@@ -577,15 +619,20 @@ extern "C" int rt_mlp_train_dev(void* net_, const double* R, const double* X,
                      hipMemcpyDeviceToDevice, s);
       hipMemcpyAsync(Xb + n * j, X + n * q, sizeof(double) * n,
                      hipMemcpyDeviceToDevice, s);
+      if (A)
+        hipMemcpyAsync(Ab + n * j, A + n * q, sizeof(double) * n,
+                       hipMemcpyDeviceToDevice, s);
       hipMemcpyAsync(taub + n * j, tau + n * q, sizeof(double) * n,
                      hipMemcpyDeviceToDevice, s);
       ekb[j] = ek[q];
       e0b[j] = e0[q];
       wb[j] = wts[q];
+      Qs[j] = Qs_all[q];
+      qc[j] = qc_all[q];
     }
     double L = 0.0;
-    st = mlp_loss_grad(net, Rb, Xb, hdiag, ekb.data(), e0b.data(), taub, wb.data(),
-                       n, batch, &L, g, Tb, s);
+    st = mlp_loss_grad(net, Rb, Xb, Ab, hdiag, ekb.data(), e0b.data(), taub, wb.data(),
+                       n, batch, Qs.data(), qc.data(), &L, g, Tb, s);
     if (st != RT_OK) break;
     if (loss_hist) loss_hist[it] = L;
     net->step += 1;
@@ -599,6 +646,7 @@ extern "C" int rt_mlp_train_dev(void* net_, const double* R, const double* X,
   (void)hipStreamSynchronize(s);
   (void)hipFree(Rb);
   (void)hipFree(Xb);
+  if (Ab) (void)hipFree(Ab);
   (void)hipFree(Tb);
   (void)hipFree(taub);
   (void)hipFree(g);
@@ -607,16 +655,20 @@ extern "C" int rt_mlp_train_dev(void* net_, const double* R, const double* X,
 
 extern "C" int rt_mlp_targets_dev(const double* X, const int* root, int ns,
                                   const double* Xfinal, int64_t n,
-                                  double min_overlap, double* tau, double* wts,
-                                  void* stream) {
+                                  double min_overlap, const double* Q,
+                                  const int64_t* q_off, const int* q_cnt, double* tau,
+                                  double* wts, void* stream) {
   RTA_CHECK(X && root && Xfinal && tau && wts, "NULL argument");
   RTA_CHECK(n > 0 && ns > 0, "empty input");
   hipStream_t s = (hipStream_t)stream;
+  std::vector<const double*> Qs;
+  std::vector<int> qc;
+  mlp_subspaces(Q, q_off, q_cnt, ns, Qs, qc);
   int* droot;
   double *ov, *sq;
   RTA_HIP(hipMalloc(&droot, sizeof(int) * ns));
   RTA_HIP(hipMalloc(&ov, sizeof(double) * ns));
-  RTA_HIP(hipMalloc(&sq, sizeof(double) * ns));
+  RTA_HIP(hipMalloc(&sq, sizeof(double) * ns * 257));  // norms + dv_sqnorms partials
   RTA_HIP(hipMemcpy(droot, root, sizeof(int) * ns, hipMemcpyHostToDevice));
   hipLaunchKernelGGL(mlp_overlap_kernel, dim3(ns), dim3(256), 0, s, X, droot, Xfinal,
                      n, ov);
@@ -624,9 +676,19 @@ extern "C" int rt_mlp_targets_dev(const double* X, const int* root, int ns,
                      ov, n, tau, sq);
   std::vector<double> hov(ns), hsq(ns), sc(ns);
   RTA_HIP(hipMemcpy(hov.data(), ov, sizeof(double) * ns, hipMemcpyDeviceToHost));
+  if (Q) {
+    // tau = x* - Q Q^T x* (x* copied in, then projected; norms recomputed)
+    for (int j = 0; j < ns; ++j)
+      RTA_HIP(hipMemcpyAsync(tau + (size_t)j * n, Xfinal + (size_t)root[j] * n,
+                             sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    for (int j = 0; j < ns; ++j)
+      if (Qs[j]) RTA_TRY(dv_project_out(Qs[j], n, qc[j], tau + (size_t)j * n, 1, n, s));
+    RTA_TRY(dv_sqnorms(tau, ns, n, sq, sq + ns, s));
+  }
   RTA_HIP(hipMemcpy(hsq.data(), sq, sizeof(double) * ns, hipMemcpyDeviceToHost));
   for (int j = 0; j < ns; ++j) {
-    const bool ok = std::fabs(hov[j]) >= min_overlap && hsq[j] > 1e-24;
+    const bool ok = std::fabs(hov[j]) >= min_overlap && hsq[j] > 1e-24 &&
+                    (!Q || Qs[j] != nullptr);
     wts[j] = ok ? 1.0 : 0.0;
     sc[j] = ok ? 1.0 / std::sqrt(hsq[j]) : 0.0;
   }

@@ -65,41 +65,67 @@ def test_qr_drops_dependent_vectors():
 
 
 # ---- learned preconditioner oracle --------------------------------------------
-def mlp_samples(n=40, ns=3, seed=5):
+def mlp_samples(n=40, ns=3, seed=5, aux=False, proj=False):
+    """Random samples (r, x0, hd, ek, e0[, a]), unit targets and, with proj,
+    an orthonormal subspace per sample that the target is orthogonal to."""
     rng = np.random.default_rng(seed)
     hd = np.sort(rng.uniform(0.2, 3.0, n))
-    out, taus = [], []
+    out, taus, Qs = [], [], []
     for s in range(ns):
         x0 = rng.standard_normal(n)
         x0 /= np.linalg.norm(x0)
         r = 0.01 * rng.standard_normal(n)
-        out.append((r, x0, hd, 0.3 + 0.1 * s, 0.3))
+        smp = (r, x0, hd, 0.3 + 0.1 * s, 0.3)
+        if aux:
+            smp = smp + (rng.standard_normal(n),)
+        out.append(smp)
         tau = rng.standard_normal(n)
+        Q = None
+        if proj:
+            Q = np.linalg.qr(np.column_stack([x0, rng.standard_normal((n, 2 + s))]))[0]
+            tau = tau - Q @ (Q.T @ tau)
+        Qs.append(Q)
         taus.append(tau / np.linalg.norm(tau))
-    return out, taus
+    return out, taus, Qs
 
 
-def test_mlp_gradient_finite_difference():
-    samples, taus = mlp_samples()
+@pytest.mark.parametrize("aux,proj", [(False, False), (True, True)])
+def test_mlp_gradient_finite_difference(aux, proj):
+    samples, taus, Qs = mlp_samples(aux=aux, proj=proj)
     rng = np.random.default_rng(9)
     p = MO.init_params(3).astype(np.float64) + 0.1 * rng.standard_normal(MO.NPARAM)
     wts = np.array([1.0, 0.5, 1.0])
-    L, g = MO.loss_grad(p, samples, taus, wts)
+    Qs = Qs if proj else None
+    L, g = MO.loss_grad(p, samples, taus, wts, Qs)
     idx = rng.choice(MO.NPARAM, size=40, replace=False)
     h = 1e-6
     for i in idx:
         e = np.zeros(MO.NPARAM)
         e[i] = h
-        fd = (MO.loss_grad(p + e, samples, taus, wts)[0] -
-              MO.loss_grad(p - e, samples, taus, wts)[0]) / (2 * h)
+        fd = (MO.loss_grad(p + e, samples, taus, wts, Qs)[0] -
+              MO.loss_grad(p - e, samples, taus, wts, Qs)[0]) / (2 * h)
         assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(g[i])) + 1e-8, i
 
 
+def test_mlp_projected_loss_ignores_subspace_components():
+    """Adding a multiple of a subspace vector to the correction changes
+    neither the projected loss nor its value's meaning (Davidson removes it)."""
+    samples, taus, Qs = mlp_samples(aux=True, proj=True)
+    p = MO.init_params(2).astype(np.float64)
+    L, _ = MO.loss_grad(p, samples, taus, np.ones(3), Qs)
+    # a network whose output is the Ritz vector x0 (in every subspace) plus the
+    # reference correction gives the same projected loss as the correction alone
+    p2 = p.copy()
+    p2[MO.H * MO.F + MO.H + MO.O * MO.H + 2] = 5.0     # c2[2]: weight of x0 sqrt(n)
+    L2, _ = MO.loss_grad(p2, samples, taus, np.ones(3), Qs)
+    assert abs(L - L2) < 1e-12
+
+
 def test_mlp_initial_network_is_reference_preconditioner():
-    samples, _ = mlp_samples()
+    samples, _, _ = mlp_samples(aux=True)
     p = MO.init_params(1)
-    for (r, x0, hd, ek, e0) in samples:
-        t, _ = MO.forward(p, r, x0, hd, ek, e0)
+    for (r, x0, hd, ek, e0, a) in samples:
+        t, _ = MO.forward(p, r, x0, hd, ek, e0, a)
         ref = DO.precond_diag(r, hd, e0)
         cos = t @ ref / (np.linalg.norm(t) * np.linalg.norm(ref))
         assert cos > 1 - 1e-12

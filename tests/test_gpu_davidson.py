@@ -103,6 +103,40 @@ def test_davidson_recorder_counts(D, problem):
     assert np.linalg.norm(A @ x - rec.e_k[q] * x - r) < 1e-12
 
 
+def test_shift_invert_matches_dense_solve(D, problem):
+    """(A - sigma)^-1 R through the Woodbury form against a dense solve."""
+    from rest_tensors_amd import davidson as DV
+    A, n, op = problem["A"], problem["n"], problem["op"]
+    si = DV.ShiftInvert(op, shift=0.05)
+    info = si.info()
+    assert abs(info["sigma"] - (problem["dvec"].min() - 0.05)) < 1e-15
+    assert info["residual"] < 1e-8
+    rng = np.random.default_rng(4)
+    for k in (1, 3, 8, 13):
+        R = rng.standard_normal((n, k))
+        dR, dT = D.DeviceArray.from_host(R), D.DeviceArray(n * k)
+        si.apply(dR, dT, k)
+        ref = np.linalg.solve(A - info["sigma"] * np.eye(n), R)
+        T = dT.to_host((n, k))
+        assert np.linalg.norm(T - ref) / np.linalg.norm(ref) < 1e-8
+    print("Newton-Schulz iterations", info["newton_schulz_iters"], "residual", info["residual"])
+
+
+def test_davidson_shift_invert_preconditioner(D, problem):
+    from rest_tensors_amd import davidson as DV
+    A, n, op, hd = problem["A"], problem["n"], problem["op"], problem["hd"]
+    dx0, _ = x0_guess(D, hd.to_host())
+    si = DV.ShiftInvert(op, shift=0.05)
+    res = DV.solve(op.matvec_fn, op.handle, dx0, hd, n, 8, tol=1e-12, maxcyc=300,
+                   maxspace=12, precond="aux", aux=si)
+    assert all(res.conv)
+    w = np.linalg.eigvalsh(A)[:8]
+    assert np.abs(res.e - w).max() < 1e-10
+    X = res.x.to_host((n, 8))
+    assert np.linalg.norm(A @ X - X * res.e[None, :], axis=0).max() < 1e-6
+    print("shift-invert iterations", res.stats["iterations"])
+
+
 def mlp_batch(D, problem, ns=6, seed=3):
     n, hd = problem["n"], problem["hd"].to_host()
     rng = np.random.default_rng(seed)
@@ -111,15 +145,16 @@ def mlp_batch(D, problem, ns=6, seed=3):
     X /= np.linalg.norm(X, axis=0)
     T = rng.standard_normal((n, ns))
     T /= np.linalg.norm(T, axis=0)
+    S = rng.standard_normal((n, ns))
     ek = 0.4 + 0.01 * np.arange(ns)
     e0 = np.full(ns, 0.4)
-    return R, X, T, ek, e0, hd
+    return R, X, T, S, ek, e0, hd
 
 
 def test_mlp_forward_matches_oracle(D, problem):
     from rest_tensors_amd import davidson as DV
     n = problem["n"]
-    R, X, _, ek, e0, hd = mlp_batch(D, problem)
+    R, X, _, S, ek, e0, hd = mlp_batch(D, problem)
     ns = R.shape[1]
     net = DV.Mlp(seed=1)
     p0 = net.get_params()
@@ -127,58 +162,93 @@ def test_mlp_forward_matches_oracle(D, problem):
     rng = np.random.default_rng(2)
     for p in (p0, p0 + 0.2 * rng.standard_normal(p0.size).astype(np.float32)):
         net.set_params(p)
-        dT = D.DeviceArray(n * ns)
-        net.apply(D.DeviceArray.from_host(R), D.DeviceArray.from_host(X), problem["hd"],
-                  ek, e0, n, ns, dT)
-        T = dT.to_host((n, ns))
-        for s in range(ns):
-            ref, _ = MO.forward(p, R[:, s], X[:, s], hd, ek[s], e0[s])
-            # FP32 network on the device, FP64 oracle
-            assert np.linalg.norm(T[:, s] - ref) / np.linalg.norm(ref) < 1e-5
+        for aux in (None, S):
+            dT = D.DeviceArray(n * ns)
+            net.apply(D.DeviceArray.from_host(R), D.DeviceArray.from_host(X), problem["hd"],
+                      ek, e0, n, ns, dT,
+                      S=None if aux is None else D.DeviceArray.from_host(aux))
+            T = dT.to_host((n, ns))
+            for s in range(ns):
+                ref, _ = MO.forward(p, R[:, s], X[:, s], hd, ek[s], e0[s],
+                                    None if aux is None else aux[:, s])
+                # FP32 network on the device, FP64 oracle
+                assert np.linalg.norm(T[:, s] - ref) / np.linalg.norm(ref) < 1e-5
 
 
 def test_mlp_gradient_matches_oracle(D, problem):
+    """Projected loss and gradient (subspaces of 2..6 vectors, one sample
+    without) with the auxiliary basis vector, against the FP64 oracle."""
     from rest_tensors_amd import davidson as DV
     n = problem["n"]
-    R, X, Tau, ek, e0, hd = mlp_batch(D, problem, ns=5, seed=8)
-    wts = np.array([1.0, 1.0, 0.0, 0.5, 1.0])
+    ns = 5
+    R, X, Tau, S, ek, e0, hd = mlp_batch(D, problem, ns=ns, seed=8)
+    rng = np.random.default_rng(6)
+    Qs, blocks, off, cnt, pos = [], [], [], [], 0
+    for s in range(ns):
+        if s == 2:
+            Qs.append(None)
+            off.append(-1)
+            cnt.append(0)
+            continue
+        m = 2 + s
+        Q = np.linalg.qr(np.column_stack([X[:, s], rng.standard_normal((n, m - 1))]))[0]
+        Tau[:, s] -= Q @ (Q.T @ Tau[:, s])
+        Tau[:, s] /= np.linalg.norm(Tau[:, s])
+        Qs.append(Q)
+        blocks.append(Q.reshape(-1, order="F"))
+        off.append(pos)
+        cnt.append(m)
+        pos += n * m
+    dQ = D.DeviceArray.from_host(np.concatenate(blocks))
+    wts = np.array([1.0, 1.0, 0.7, 0.5, 1.0])
     net = DV.Mlp(seed=4)
-    p = net.get_params() + 0.1 * np.random.default_rng(5).standard_normal(708).astype(np.float32)
+    p = net.get_params() + 0.1 * np.random.default_rng(5).standard_normal(
+        net.nparam).astype(np.float32)
     net.set_params(p)
     L, g = net.loss_grad(D.DeviceArray.from_host(R), D.DeviceArray.from_host(X),
-                         problem["hd"], ek, e0, D.DeviceArray.from_host(Tau), wts, n, 5)
-    samples = [(R[:, s], X[:, s], hd, ek[s], e0[s]) for s in range(5)]
-    Lr, gr = MO.loss_grad(p, samples, [Tau[:, s] for s in range(5)], wts)
+                         problem["hd"], ek, e0, D.DeviceArray.from_host(Tau), wts, n, ns,
+                         S=D.DeviceArray.from_host(S), q=(dQ, off, cnt))
+    samples = [(R[:, s], X[:, s], hd, ek[s], e0[s], S[:, s]) for s in range(ns)]
+    Lr, gr = MO.loss_grad(p, samples, [Tau[:, s] for s in range(ns)], wts, Qs)
     assert abs(L - Lr) < 1e-5 * max(1.0, abs(Lr))
     assert np.linalg.norm(g - gr) / np.linalg.norm(gr) < 1e-3
 
 
 def test_mlp_training_and_preconditioned_solve(D, problem):
-    """Train on plain-Davidson samples of this problem, then solve with the
-    network: converged to the same eigenvalues, residual still the judge."""
+    """Record plain-Davidson samples (with their subspaces), train the network
+    on the GPU with the shift-invert correction in its basis, then solve with
+    it: same eigenvalues, the residual still the judge, and fewer iterations
+    than the plain diagonal preconditioner."""
     from rest_tensors_amd import davidson as DV
     A, n, op, hd = problem["A"], problem["n"], problem["op"], problem["hd"]
     dx0, _ = x0_guess(D, hd.to_host())
-    rec = DV.Recorder(n, 600)
+    si = DV.ShiftInvert(op, shift=0.05)
+    rec = DV.Recorder(n, 600, subspace_doubles=n * 40 * 120)
     plain = DV.solve(op.matvec_fn, op.handle, dx0, hd, n, 8, tol=1e-12, maxcyc=300,
                      maxspace=12, recorder=rec)
     ns = rec.count
+    assert (rec.xs_off[:ns] >= 0).sum() > 10
     tau = D.DeviceArray(n * ns)
-    wts = DV.mlp_targets(rec.x, rec.root[:ns], ns, plain.x, n, tau)
+    wts = DV.mlp_targets(rec.x, rec.root[:ns], ns, plain.x, n, tau, recorder=rec)
     assert wts.sum() > 10
+    Sa = D.DeviceArray(n * ns)
+    si.apply(rec.r, Sa, ns)
     net = DV.Mlp(seed=1)
-    L0, _ = net.loss_grad(rec.r, rec.x, hd, rec.e_k[:ns], rec.e_0[:ns], tau, wts, n, ns)
-    hist = net.train(rec.r, rec.x, hd, rec.e_k[:ns], rec.e_0[:ns], tau, wts, n, ns,
-                     steps=150, batch=32, lr=3e-3)
-    L1, _ = net.loss_grad(rec.r, rec.x, hd, rec.e_k[:ns], rec.e_0[:ns], tau, wts, n, ns)
+    args = (rec.r, rec.x, hd, rec.e_k[:ns], rec.e_0[:ns], tau, wts, n, ns)
+    L0, _ = net.loss_grad(*args, S=Sa, recorder=rec)
+    hist = net.train(*args, steps=300, batch=32, lr=3e-3, S=Sa, recorder=rec)
+    L1, _ = net.loss_grad(*args, S=Sa, recorder=rec)
     print("training loss", L0, "->", L1, "(first/last minibatch", hist[0], hist[-1], ")")
     assert L1 < L0
     res = DV.solve(op.matvec_fn, op.handle, dx0, hd, n, 8, tol=1e-12, maxcyc=300,
-                   maxspace=12, precond="mlp", mlp=net)
+                   maxspace=12, precond="mlp", mlp=net, aux=si)
     assert all(res.conv)
     w = np.linalg.eigvalsh(A)[:8]
     assert np.abs(res.e - w).max() < 1e-10
     X = res.x.to_host((n, 8))
     assert np.linalg.norm(A @ X - X * res.e[None, :], axis=0).max() < 1e-6
-    print("plain iterations", plain.stats["iterations"], "mlp iterations",
-          res.stats["iterations"])
+    sio = DV.solve(op.matvec_fn, op.handle, dx0, hd, n, 8, tol=1e-12, maxcyc=300,
+                   maxspace=12, precond="aux", aux=si)
+    print("iterations: plain", plain.stats["iterations"], "shift-invert only",
+          sio.stats["iterations"], "mlp", res.stats["iterations"])
+    assert res.stats["iterations"] < plain.stats["iterations"]
