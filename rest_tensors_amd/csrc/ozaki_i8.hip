@@ -189,37 +189,90 @@ __global__ __launch_bounds__(256) void rik_oz_i8_kernel(OzParams g) {
   const int arow = (wave_m * 64 + l15) * UPR, brow = (wave_n * 64 + l15) * UPR;
   const int kx = kg ^ (l15 & 3), hx = l15 >> 2;
 
-  if (niter > 0) issue(0);
+  // L2 prefetch of the slab PF k-chunks ahead: each lane touches one 128-byte
+  // line (2 per row of 256 bytes, 512 lines for both operands, 2 loads per
+  // wave) through global_load_lds into a 1 KB per-wave scratch area of LDS
+  // that is never read, so no VGPR is tied up by a load in flight.
+  constexpr int PF = 3;
+  int64_t pP = b0, pkc = 0;  // next (P, kc) to prefetch
+  auto prefetch = [&]() {
+    const char* slab = (const char*)g.S + pP * pstride + pkc * 256;
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const int line = (wave * 2 + q) * 64 + lane;  // 0..511
+      const int op = line >> 8, row = (line & 255) >> 1, half = line & 1;
+      int64_t i = (op == 0 ? m0 : n0) + row;
+      i = i < g.nmo ? i : g.nmo - 1;
+      __builtin_amdgcn_global_load_lds((const oz_glb_t*)(slab + i * g.nkc * 256 + half * 128),
+                                       (oz_lds_t*)(oz_lds + 2 * STAGE + wave * 64), 16, 0, 0);
+    }
+    if (++pkc == g.nkc) {
+      pkc = 0;
+      ++pP;
+    }
+  };
+  // k-chunk j is prefetched in iteration j - PF (the prologue covers 1..PF-1)
+  if (niter > 0) {
+    for (int j = 0; j < PF && j < niter; ++j) {
+      if (j > 0) prefetch();
+      else if (++pkc == g.nkc) {  // chunk 0 is loaded for real right away
+        pkc = 0;
+        ++pP;
+      }
+    }
+    issue(0);
+  }
   for (int64_t it = 0; it < niter; ++it) {
-    // stage it landed (this wave's loads), then every wave's loads landed and
-    // every wave finished reading stage it-1, whose buffer is refilled next
-    wait_vmcnt<0>();
+    // stage it landed (this wave's loads; the prefetch issued after it may
+    // still be in flight), then every wave's loads landed and every wave
+    // finished reading stage it-1, whose buffer is refilled next
+    if (it >= 1 && it - 1 + PF < niter) wait_vmcnt<2>();
+    else wait_vmcnt<0>();
     __builtin_amdgcn_s_barrier();
     asm volatile("" ::: "memory");  // no LDS access moves across the barrier
     const uint4* As = oz_lds + (it & 1) * STAGE;
     const uint4* Bs = As + OPU;
-    i32x4_t a[4][4];
-#pragma unroll
-    for (int s = 0; s < 4; ++s)
+    i32x4_t a[4][4], b[4][4];
+    auto load_a = [&](int s) {
 #pragma unroll
       for (int i = 0; i < 4; ++i)
         a[s][i] = __builtin_bit_cast(i32x4_t, As[arow + i * 16 * UPR + ((s ^ hx) * 4 + kx)]);
-    if (it + 1 < niter) issue((int)((it + 1) & 1));
-#pragma unroll
-    for (int t = 0; t < 4; ++t) {
-      i32x4_t b[4];
+    };
+    auto load_b = [&](int t) {
 #pragma unroll
       for (int j = 0; j < 4; ++j)
-        b[j] = __builtin_bit_cast(i32x4_t, Bs[brow + j * 16 * UPR + ((t ^ hx) * 4 + kx)]);
+        b[t][j] = __builtin_bit_cast(i32x4_t, Bs[brow + j * 16 * UPR + ((t ^ hx) * 4 + kx)]);
+    };
+    auto prod = [&](int s, int t) {
 #pragma unroll
-      for (int s = 0; s + t < 4; ++s)
+      for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int i = 0; i < 4; ++i)
-#pragma unroll
-          for (int j = 0; j < 4; ++j)
-            acc[s + t][i][j] =
-                __builtin_amdgcn_mfma_i32_16x16x64_i8(b[j], a[s][i], acc[s + t][i][j], 0, 0, 0);
-    }
+        for (int j = 0; j < 4; ++j)
+          acc[s + t][i][j] =
+              __builtin_amdgcn_mfma_i32_16x16x64_i8(b[t][j], a[s][i], acc[s + t][i][j], 0, 0, 0);
+    };
+    // slices enter in pairs (a_s, b_s); each pair unlocks the products that
+    // need it, so the first MFMAs wait for 8 LDS reads, not 20
+    load_a(0);
+    load_b(0);
+    if (it + 1 < niter) issue((int)((it + 1) & 1));
+    if (it + PF < niter) prefetch();
+    load_a(1);
+    load_b(1);
+    prod(0, 0);
+    load_a(2);
+    load_b(2);
+    prod(0, 1);
+    prod(1, 0);
+    prod(1, 1);
+    load_a(3);
+    load_b(3);
+    prod(0, 2);
+    prod(2, 0);
+    prod(1, 2);
+    prod(2, 1);
+    prod(0, 3);
+    prod(3, 0);
   }
 
   // fold: K_part = 2^(e_i+e_j) / 127^2 * sum_q 254^-q L_q (all zero if niter == 0)
@@ -297,7 +350,7 @@ int rik_ozaki_i8(const double* B, int64_t nocc, int64_t nmo, int64_t naux, doubl
       std::max<int64_t>(1, std::min<int64_t>(ngroups, kOzMaxSlotBytes / slot_bytes));
   double* part = nullptr;
   RTA_TRY(scratch_get(SCRATCH_GEMM_PARTIAL, (size_t)slot_bytes * per_launch, (void**)&part));
-  const size_t lds = 2 * 2 * 128 * 16 * 16;
+  const size_t lds = 2 * 2 * 128 * 16 * 16 + 4 * 1024;  // 2 stages + prefetch scratch
   static bool attr = false;
   if (!attr) {
     RTA_HIP(hipFuncSetAttribute((const void*)rik_oz_i8_kernel,
