@@ -149,6 +149,27 @@ int64_t rt_ozaki_i8_group_size(int64_t nocc, int64_t nmo, int64_t naux);
 int rt_exchange_energy_dev(const double* B, int64_t nocc, int64_t nmo,
                            int64_t naux, double* e_x, void* stream);
 
+/* ---- level-2 / rank-k primitives REST composes on RI tensors (A7) ------- */
+/* y = alpha op(A) x + beta y, A [m, n] column-major (lda >= m); op 'N' or
+ * 'T'/'C'; positive increments.  Replaces _dgemv
+ * (src/matrix/matrix_blas_lapack.rs:38-70); e.g. gamma = A^T d and J = A gamma
+ * on the packed ri3ao of rt_rifull_to_matfull_symm_dev. */
+int rt_dgemv_dev(char trans, int64_t m, int64_t n, double alpha, const double* A,
+                 int64_t lda, const double* x, int64_t incx, double beta, double* y,
+                 int64_t incy, void* stream);
+/* C = alpha A A^T + beta C ('N', A [n, k]) or alpha A^T A + beta C ('T',
+ * A [k, n]); only the uplo ('U'/'L') triangle of C is read and written.
+ * Replaces _dsyrk (src/matrix/matrix_blas_lapack.rs:392-413). */
+int rt_dsyrk_dev(char uplo, char trans, int64_t n, int64_t k, double alpha,
+                 const double* A, int64_t lda, double beta, double* C, int64_t ldc,
+                 void* stream);
+/* out[p] = sum_i A[i, p] B[i, p] for p < min(a_cols, b_cols), i <
+ * min(a_rows, b_rows) (zeros when a row count is 0).  Replaces the einsum
+ * "ip,ip->p" of _einsum_02_* (src/matrix/matrix_blas_lapack.rs:1293-1349). */
+int rt_einsum_ip_ip_p_dev(const double* A, int64_t a_rows, int64_t a_cols,
+                          const double* B, int64_t b_rows, int64_t b_cols,
+                          double* out, void* stream);
+
 /* ---- packed-upper MatrixUpper (A8/A9) ---------------------------------- */
 /* Packed index of (i,j), i<=j: j*(j+1)/2 + i (index.rs:209-233).
  * unpack: full[n,n] from packed[len]; len must be triangular (else

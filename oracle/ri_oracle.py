@@ -27,6 +27,13 @@ Pinning summary:
   general_dgemm       pinned: golden vectors of oracle/_ref and the doc values
                       of matrix_blas_lapack.rs:91-119.
   copy_mm/mr/rm/rr    pinned: golden vectors of oracle/_ref.
+  dgemv / dsyrk       pinned: scipy.linalg.blas (OpenBLAS, the BLAS the
+                      reference links), incl. strides and the untouched triangle;
+                      RI-J composition on the packed layout equals rij.
+  einsum_ip_ip_p      pinned: explicit column dots, zero-row case.
+  ozaki_i8 (slices, K) no reference counterpart (restates the INT8 form);
+                      pinned: reconstruction bound, brute-force level sums,
+                      1e-8 against rik.
   special_dgemm_01    parity unpinned against the reference (its Fortran
                       aliases dgemm's A and C, restmatr.f90:140-150); pinned
                       against the product formula it documents instead.
@@ -216,6 +223,47 @@ def rik_ozaki_i8(B: np.ndarray, gp: int) -> np.ndarray:
             v = L[q] + v * (1.0 / 254.0)
         K += np.ldexp(v * (1.0 / 16129.0), E[g][:, None] + E[g][None, :])
     return np.asfortranarray(K)
+
+
+# ---- level-2 / rank-k primitives ------------------------------------------------
+def dgemv(A, x, y, trans, alpha, beta, incx=1, incy=1):
+    """_dgemv (src/matrix/matrix_blas_lapack.rs:38-70): y = alpha op(A) x + beta y
+    on strided vectors, with the reference's length check
+    len = 1 + (len-1)*|inc| (it panics otherwise; here ValueError)."""
+    m, n = A.shape
+    t = trans.lower() != "n"
+    lx, ly = (m, n) if t else (n, m)
+    if len(x) != 1 + (lx - 1) * abs(incx) or len(y) != 1 + (ly - 1) * abs(incy):
+        raise ValueError("dgemv shape mismatch")
+    out = np.array(y, dtype=np.float64, copy=True)
+    prod = (A.T if t else A) @ np.asarray(x)[::incx]
+    out[::incy] = alpha * prod + beta * out[::incy]
+    return out
+
+
+def dsyrk(A, Cm, uplo, trans, alpha, beta):
+    """_dsyrk (src/matrix/matrix_blas_lapack.rs:392-413): the uplo triangle of
+    C (diagonal included) becomes alpha op(A) op(A)^T + beta C with op(A) = A
+    for 'N', A^T for 'T'; the other triangle is left as it was."""
+    P = A @ A.T if trans.lower() == "n" else A.T @ A
+    out = np.array(Cm, dtype=np.float64, copy=True, order="F")
+    n = out.shape[0]
+    mask = np.triu(np.ones((n, n), bool)) if uplo.lower() == "u" else np.tril(np.ones((n, n), bool))
+    out[mask] = alpha * P[mask] + beta * out[mask]
+    return out
+
+
+def einsum_ip_ip_p(A, B):
+    """einsum "ip,ip->p" (_einsum_02_general, src/matrix/matrix_blas_lapack.rs:
+    1324-1349): column dots over min(rows) entries for min(cols) columns, zeros
+    when a matrix has no rows."""
+    ar, ac = A.shape
+    br, bc = B.shape
+    npp = min(ac, bc)
+    if ar == 0 or br == 0:
+        return np.zeros(npp)
+    r = min(ar, br)
+    return np.einsum("ip,ip->p", A[:r, :npp], B[:r, :npp])
 
 
 # ---- packed upper ---------------------------------------------------------------

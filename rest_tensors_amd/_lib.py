@@ -58,6 +58,9 @@ RT_SIGNATURES = {
     "rt_rik_dev": (i32, [vp, i64, i64, i64, vp, pf64, i32, vp]),
     "rt_exchange_energy_dev": (i32, [vp, i64, i64, i64, pf64, vp]),
     "rt_ozaki_i8_group_size": (i64, [i64, i64, i64]),
+    "rt_dgemv_dev": (i32, [ch, i64, i64, f64, vp, i64, vp, i64, f64, vp, i64, vp]),
+    "rt_dsyrk_dev": (i32, [ch, ch, i64, i64, f64, vp, i64, f64, vp, i64, vp]),
+    "rt_einsum_ip_ip_p_dev": (i32, [vp, i64, i64, vp, i64, i64, vp, vp]),
     "rt_upper_unpack_dev": (i32, [vp, i64, vp, vp]),
     "rt_upper_pack_dev": (i32, [vp, i64, vp, vp]),
     "rt_rifull_to_matfull_symm_dev": (i32, [vp, i64, i64, vp, vp]),

@@ -207,3 +207,22 @@ def dgemm(opa, opb, m, n, k, alpha, A: DeviceArray, lda, B: DeviceArray, ldb, be
           Cm: DeviceArray, ldc, stream=None):
     check(lib().rt_dgemm_dev(opa.encode(), opb.encode(), m, n, k, alpha, A.ptr, lda,
                              B.ptr, ldb, beta, Cm.ptr, ldc, stream))
+
+
+# ---- level-2 / rank-k primitives (A7) -------------------------------------------
+def dgemv(trans, m, n, alpha, A: DeviceArray, lda, x: DeviceArray, beta, y: DeviceArray,
+          incx=1, incy=1, stream=None):
+    check(lib().rt_dgemv_dev(trans.encode(), m, n, alpha, A.ptr, lda, x.ptr, incx, beta,
+                             y.ptr, incy, stream))
+
+
+def dsyrk(uplo, trans, n, k, alpha, A: DeviceArray, lda, beta, Cm: DeviceArray, ldc,
+          stream=None):
+    check(lib().rt_dsyrk_dev(uplo.encode(), trans.encode(), n, k, alpha, A.ptr, lda, beta,
+                             Cm.ptr, ldc, stream))
+
+
+def einsum_ip_ip_p(A: DeviceArray, a_rows, a_cols, B: DeviceArray, b_rows, b_cols,
+                   out: DeviceArray, stream=None):
+    check(lib().rt_einsum_ip_ip_p_dev(A.ptr, a_rows, a_cols, B.ptr, b_rows, b_cols,
+                                      out.ptr, stream))

@@ -295,3 +295,57 @@ def test_ozaki_i8_k_accuracy():
     Bb = bad * np.ldexp(1.0, np.repeat(E, 4, axis=0).T)[None, :, :64]
     Kb, _, _ = O.rik(Bb)
     assert np.linalg.norm(Kb - K) / np.linalg.norm(K) > 1e-6
+
+
+# ---- level-2 / rank-k primitives against the BLAS the reference links -----------
+def test_dgemv_matches_blas():
+    from scipy.linalg import blas
+    rng = np.random.default_rng(21)
+    A = np.asfortranarray(rng.standard_normal((37, 23)))
+    for trans, lx, ly in (("N", 23, 37), ("T", 37, 23)):
+        for incx, incy in ((1, 1), (2, 3)):
+            x = rng.standard_normal(1 + (lx - 1) * incx)
+            y = rng.standard_normal(1 + (ly - 1) * incy)
+            ref = blas.dgemv(0.7, A, x, beta=-1.3, y=y.copy(), incx=incx, incy=incy,
+                             trans=0 if trans == "N" else 1)
+            assert np.allclose(O.dgemv(A, x, y, trans, 0.7, -1.3, incx, incy), ref,
+                               rtol=1e-14, atol=1e-13)
+    with pytest.raises(ValueError):
+        O.dgemv(A, np.zeros(22), np.zeros(37), "N", 1.0, 0.0)
+
+
+def test_dsyrk_matches_blas_and_keeps_other_triangle():
+    from scipy.linalg import blas
+    rng = np.random.default_rng(22)
+    for trans, shape in (("N", (19, 7)), ("T", (7, 19))):
+        A = np.asfortranarray(rng.standard_normal(shape))
+        for uplo in ("U", "L"):
+            Cm = np.asfortranarray(rng.standard_normal((19, 19)))
+            ref = blas.dsyrk(0.5, A, beta=2.0, c=Cm.copy(), trans=0 if trans == "N" else 1,
+                             lower=1 if uplo == "L" else 0)
+            out = O.dsyrk(A, Cm, uplo, trans, 0.5, 2.0)
+            assert np.allclose(out, ref, rtol=1e-14, atol=1e-13)
+            other = np.tril(np.ones((19, 19), bool), -1) if uplo == "U" else \
+                np.triu(np.ones((19, 19), bool), 1)
+            assert np.array_equal(out[other], Cm[other])
+
+
+def test_einsum_ip_ip_p_and_rij_composition():
+    rng = np.random.default_rng(23)
+    A, B = rng.standard_normal((9, 5)), rng.standard_normal((7, 6))
+    out = O.einsum_ip_ip_p(A, B)
+    assert out.shape == (5,)
+    assert np.allclose(out, [A[:7, p] @ B[:7, p] for p in range(5)], rtol=1e-15)
+    assert np.array_equal(O.einsum_ip_ip_p(np.zeros((0, 4)), B), np.zeros(4))
+    # RI-J as REST composes it on the packed ri3ao: gamma = A^T (w o d), J = A gamma
+    nbas, naux = 12, 20
+    A3 = O.gen_ri3ao(nbas, naux, seed=31)
+    Dm = O.density(O.gen_orthonormal(nbas, seed=32), 4)
+    Ap = O.rifull_to_matfull_symm(A3)
+    w = 2.0 - np.eye(nbas)                           # off-diagonal pairs twice
+    dw = O.upper_pack(w * Dm)
+    gamma = O.dgemv(Ap, dw, np.zeros(naux), "T", 1.0, 0.0)
+    Jp = O.dgemv(Ap, gamma, np.zeros(Ap.shape[0]), "N", 1.0, 0.0)
+    gamma_ref, J, e_j = O.rij(A3, Dm)
+    assert np.allclose(gamma, gamma_ref, rtol=1e-13)
+    assert np.allclose(O.upper_unpack(Jp), J, rtol=1e-13, atol=1e-13)
