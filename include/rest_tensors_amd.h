@@ -42,6 +42,10 @@ extern "C" {
 
 /* ---- runtime plumbing -------------------------------------------------- */
 const char* rt_last_error(void);
+/* Empties this thread's last-error message (so that a caller of the void
+ * drop-in symbols, which have no status channel, can tell whether a call
+ * failed: the message is non-empty afterwards). */
+void rt_clear_error(void);
 int rt_version(void);
 int rt_device_count(int* count);
 int rt_set_device(int device);

@@ -21,6 +21,7 @@ ch = C.c_char
 # name -> (restype, argtypes) for the extended ABI (include/rest_tensors_amd.h)
 RT_SIGNATURES = {
     "rt_last_error": (cp, []),
+    "rt_clear_error": (None, []),
     "rt_version": (i32, []),
     "rt_device_count": (i32, [pi32]),
     "rt_set_device": (i32, [i32]),

@@ -57,7 +57,29 @@ def build(verbose: bool = True) -> str:
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB} from {len(srcs)} sources", flush=True)
+    build_host_mirror_test(verbose)
     return LIB
+
+
+REPO = os.path.dirname(HERE)
+MIRROR_TEST_SRC = os.path.join(REPO, "tests", "cpp", "test_host_mirror.cpp")
+MIRROR_TEST_BIN = os.path.join(OBJ, "test_host_mirror")
+
+
+def build_host_mirror_test(verbose: bool = True) -> str | None:
+    """Compiles the C++ host-mirror test (include/rest_tensors.hpp) with g++
+    against the library; the binary sits next to the objects."""
+    if not os.path.exists(MIRROR_TEST_SRC):
+        return None
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-I", os.path.join(REPO, "include"),
+           MIRROR_TEST_SRC, "-L", HERE, "-lrest_tensors_amd",
+           "-Wl,-rpath,$ORIGIN/..", "-o", MIRROR_TEST_BIN]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"host mirror test build failed: {' '.join(cmd)}\n{r.stderr}")
+    if verbose:
+        print(f"built {MIRROR_TEST_BIN}", flush=True)
+    return MIRROR_TEST_BIN
 
 
 if __name__ == "__main__":

@@ -114,6 +114,7 @@ using namespace rta;
 extern "C" {
 
 const char* rt_last_error(void) { return g_last_error.c_str(); }
+void rt_clear_error(void) { g_last_error.clear(); }
 
 int rt_version(void) { return 100; }
 
