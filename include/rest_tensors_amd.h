@@ -174,6 +174,31 @@ int rt_einsum_ip_ip_p_dev(const double* A, int64_t a_rows, int64_t a_cols,
                           const double* B, int64_t b_rows, int64_t b_cols,
                           double* out, void* stream);
 
+/* ---- FP64 aux-metric functions (A11) ------------------------------------ */
+/* Cholesky factorization of the SPD matrix whose uplo ('U'/'L') triangle A
+ * holds: A = U^T U or L L^T, the factor written over that triangle, the other
+ * strict triangle untouched; *info (host) = 0, or k > 0 when the leading
+ * minor of order k is not positive definite (the reference panics then).
+ * Replaces _dpotrf (src/matrix/matrix_blas_lapack.rs:518-534).  Synchronizes. */
+int rt_dpotrf_dev(char uplo, int64_t n, double* A, int64_t lda, int* info,
+                  void* stream);
+/* out = the Cholesky factor of A^-1 (lower for 'L': A^-1 = M M^T, upper for
+ * 'U': A^-1 = U^T U), the other strict triangle zero; A read from its uplo
+ * triangle.  RT_ERR_NUMERIC when A is not positive definite (the reference
+ * returns None).  Replaces MatrixFull::cholesky_decompose_inverse
+ * (src/matrix/matrix_blas_lapack.rs:931-1002).  Synchronizes. */
+int rt_cholesky_decompose_inverse_dev(char uplo, int64_t n, const double* A,
+                                      int64_t lda, double* out, int64_t ldo,
+                                      void* stream);
+/* out = A^p for symmetric A (lower triangle read) and p in {1, -1, 1/2, -1/2}
+ * (inverse through Cholesky; square roots through the polar factor of the
+ * Cholesky factor, Newton-Schulz on FP64 GEMMs).  The reference (MatrixFull::lapack_power,
+ * src/matrix/matrix_blas_lapack.rs:1004-1064) drops eigenvalues below
+ * `threshold`; here every eigenvalue must lie above it (checked by factoring
+ * A - threshold I), else RT_ERR_ARG.  Synchronizes. */
+int rt_matrix_power_dev(int64_t n, const double* A, int64_t lda, double p,
+                        double threshold, double* out, int64_t ldo, void* stream);
+
 /* ---- packed-upper MatrixUpper (A8/A9) ---------------------------------- */
 /* Packed index of (i,j), i<=j: j*(j+1)/2 + i (index.rs:209-233).
  * unpack: full[n,n] from packed[len]; len must be triangular (else

@@ -62,6 +62,9 @@ RT_SIGNATURES = {
     "rt_dgemv_dev": (i32, [ch, i64, i64, f64, vp, i64, vp, i64, f64, vp, i64, vp]),
     "rt_dsyrk_dev": (i32, [ch, ch, i64, i64, f64, vp, i64, f64, vp, i64, vp]),
     "rt_einsum_ip_ip_p_dev": (i32, [vp, i64, i64, vp, i64, i64, vp, vp]),
+    "rt_dpotrf_dev": (i32, [ch, i64, vp, i64, pi32, vp]),
+    "rt_cholesky_decompose_inverse_dev": (i32, [ch, i64, vp, i64, vp, i64, vp]),
+    "rt_matrix_power_dev": (i32, [i64, vp, i64, f64, f64, vp, i64, vp]),
     "rt_upper_unpack_dev": (i32, [vp, i64, vp, vp]),
     "rt_upper_pack_dev": (i32, [vp, i64, vp, vp]),
     "rt_rifull_to_matfull_symm_dev": (i32, [vp, i64, i64, vp, vp]),

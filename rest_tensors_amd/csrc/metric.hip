@@ -1,0 +1,394 @@
+// FP64 aux-metric functions kept in FP64 on the RI path (A11):
+//   _dpotrf                     src/matrix/matrix_blas_lapack.rs:518-534
+//   cholesky_decompose_inverse  src/matrix/matrix_blas_lapack.rs:931-1002
+//   lapack_power (p = 1, -1, +-1/2)  src/matrix/matrix_blas_lapack.rs:1004-1064
+// Blocked right-looking Cholesky with 64-wide panels: the diagonal block is
+// factored by one workgroup in LDS, the panel below by a row-parallel
+// triangular solve, the trailing matrix by the FP64 MFMA GEMM.  Inverses go
+// through the triangular inverse (blocked forward substitution) and one TN
+// GEMM; square roots through the polar factor of the Cholesky factor
+// (Newton-Schulz on GEMMs; stable, and it sees only sqrt(cond A)).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "common.hpp"
+#include "dvkernels.hpp"
+#include "timer.hpp"
+
+namespace rta {
+
+constexpr int CH_NB = 64;
+
+static unsigned mt_grid(int64_t len) { return (unsigned)((len + 255) / 256); }
+
+// One workgroup of 64 threads factors W[k0:k0+kb, k0:k0+kb] (lower) in LDS.
+// info (device): first failing pivot (1-based) if still 0.
+__global__ __launch_bounds__(CH_NB) void potf2_kernel(double* __restrict__ W, int64_t ld,
+                                                      int64_t k0, int kb,
+                                                      int* __restrict__ info) {
+  __shared__ double L[CH_NB][CH_NB + 1];
+  const int t = threadIdx.x;
+  for (int j = 0; j < CH_NB; ++j)
+    L[t][j] = (t < kb && j < kb) ? W[(k0 + t) + ld * (k0 + j)] : (t == j ? 1.0 : 0.0);
+  for (int j = 0; j < kb; ++j) {
+    __syncthreads();
+    if (t == j) {
+      const double d = L[j][j];
+      if (!(d > 0.0) && *info == 0) *info = (int)(k0 + j + 1);
+      L[j][j] = sqrt(d);
+    }
+    __syncthreads();
+    if (t > j && t < kb) L[t][j] /= L[j][j];
+    __syncthreads();
+    if (t > j && t < kb)
+      for (int c = j + 1; c <= t; ++c) L[t][c] -= L[t][j] * L[c][j];
+  }
+  __syncthreads();
+  if (t < kb)
+    for (int j = 0; j <= t; ++j) W[(k0 + t) + ld * (k0 + j)] = L[t][j];
+}
+
+// Rows r in [r0, r1): W[r, k0:k0+kb] <- W[r, k0:k0+kb] L11^-T (L11 = the
+// factored diagonal block), i.e. x_j = (a_j - sum_{i<j} x_i L11[j][i]) / L11[j][j].
+// 64 threads, one row each; the block's rows and L11 live in LDS.
+__global__ __launch_bounds__(CH_NB) void trsm_rows_kernel(double* __restrict__ W, int64_t ld,
+                                                          int64_t k0, int kb, int64_t r0,
+                                                          int64_t r1) {
+  __shared__ double L[CH_NB][CH_NB + 1];
+  __shared__ double xs[CH_NB][CH_NB + 1];
+  const int t = threadIdx.x;
+  const int64_t rb = r0 + blockIdx.x * (int64_t)CH_NB;
+  for (int j = 0; j < CH_NB; ++j) {
+    L[t][j] = (t < kb && j < kb) ? W[(k0 + t) + ld * (k0 + j)] : (t == j ? 1.0 : 0.0);
+    xs[t][j] = (rb + t < r1 && j < kb) ? W[(rb + t) + ld * (k0 + j)] : 0.0;
+  }
+  __syncthreads();
+  if (rb + t >= r1) return;
+  for (int j = 0; j < kb; ++j) {
+    double v = xs[t][j];
+    for (int i = 0; i < j; ++i) v -= xs[t][i] * L[j][i];
+    xs[t][j] = v / L[j][j];
+  }
+  for (int j = 0; j < kb; ++j) W[(rb + t) + ld * (k0 + j)] = xs[t][j];
+}
+
+// Columns c in [0, nc): X[k0:k0+kb, c] <- L11^-1 X[k0:k0+kb, c] (forward
+// substitution with the diagonal block of the factor Lf); 64 columns per
+// workgroup, one per thread.
+__global__ __launch_bounds__(CH_NB) void trsm_cols_kernel(const double* __restrict__ Lf,
+                                                          int64_t ldl, double* __restrict__ X,
+                                                          int64_t ldx, int64_t k0, int kb,
+                                                          int64_t nc) {
+  __shared__ double L[CH_NB][CH_NB + 1];
+  __shared__ double ys[CH_NB][CH_NB + 1];
+  const int t = threadIdx.x;
+  const int64_t cb = blockIdx.x * (int64_t)CH_NB;
+  for (int j = 0; j < CH_NB; ++j)
+    L[t][j] = (t < kb && j < kb) ? Lf[(k0 + t) + ldl * (k0 + j)] : (t == j ? 1.0 : 0.0);
+  // ys[c][j] = X[k0 + j, cb + c]: thread t reads row k0+t across the 64 columns
+  for (int c = 0; c < CH_NB; ++c)
+    ys[c][t] = (t < kb && cb + c < nc) ? X[(k0 + t) + ldx * (cb + c)] : 0.0;
+  __syncthreads();
+  if (cb + t < nc) {
+    for (int j = 0; j < kb; ++j) {
+      double v = ys[t][j];
+      for (int i = 0; i < j; ++i) v -= L[j][i] * ys[t][i];
+      ys[t][j] = v / L[j][j];
+    }
+  }
+  __syncthreads();
+  for (int c = 0; c < CH_NB; ++c)
+    if (t < kb && cb + c < nc) X[(k0 + t) + ldx * (cb + c)] = ys[c][t];
+}
+
+// S (n x n, ld n) = the symmetric matrix whose uplo triangle is A's
+__global__ void mt_symmetrize_kernel(const double* __restrict__ A, int64_t lda, int64_t n,
+                                     bool upper, double* __restrict__ S) {
+  const int64_t e = blockIdx.x * (int64_t)256 + threadIdx.x;
+  if (e >= n * n) return;
+  const int64_t i = e % n, j = e / n;
+  const bool in = upper ? (i <= j) : (i >= j);
+  S[e] = in ? A[i + lda * j] : A[j + lda * i];
+}
+
+// out (ld ldo) <- the lower triangle of W (ld n) or its transpose into the
+// upper triangle; zero_other: write zeros in the other strict triangle
+__global__ void mt_store_tri_kernel(const double* __restrict__ W, int64_t n, bool upper,
+                                    bool zero_other, double* __restrict__ out, int64_t ldo) {
+  const int64_t e = blockIdx.x * (int64_t)256 + threadIdx.x;
+  if (e >= n * n) return;
+  const int64_t i = e % n, j = e / n;
+  if (upper) {
+    if (i <= j) out[i + ldo * j] = W[j + n * i];
+    else if (zero_other) out[i + ldo * j] = 0.0;
+  } else {
+    if (i >= j) out[i + ldo * j] = W[i + n * j];
+    else if (zero_other) out[i + ldo * j] = 0.0;
+  }
+}
+
+__global__ void mt_fill_kernel(double* __restrict__ X, int64_t len, double v) {
+  const int64_t e = blockIdx.x * (int64_t)256 + threadIdx.x;
+  if (e < len) X[e] = v;
+}
+
+// rows k0..k0+kb of X (ld n), columns [k0, k0+kb): identity block
+__global__ void mt_eye_block_kernel(double* __restrict__ X, int64_t n, int64_t k0, int kb) {
+  const int t = threadIdx.x;
+  if (t < kb) X[(k0 + t) + n * (k0 + t)] = 1.0;
+}
+
+// M <- (M + M^T)/2 in place (each pair handled by the thread of its lower element)
+__global__ void mt_sym_avg_kernel(double* __restrict__ M, int64_t n) {
+  const int64_t e = blockIdx.x * (int64_t)256 + threadIdx.x;
+  if (e >= n * n) return;
+  const int64_t i = e % n, j = e / n;
+  if (i <= j) return;
+  const double v = 0.5 * (M[i + n * j] + M[j + n * i]);
+  M[i + n * j] = v;
+  M[j + n * i] = v;
+}
+
+// B = a * A + b * I (n x n)
+__global__ void mt_axpi_kernel(const double* __restrict__ A, int64_t n, double a, double b,
+                               double* __restrict__ B) {
+  const int64_t e = blockIdx.x * (int64_t)256 + threadIdx.x;
+  if (e >= n * n) return;
+  B[e] = a * A[e] + ((e % n == e / n) ? b : 0.0);
+}
+
+// W (ld n) <- lower Cholesky factor of W in its lower triangle (the strict
+// upper triangle is left holding intermediate values); info (host) = first
+// non-positive pivot (1-based) or 0.
+static int chol_lower(double* W, int64_t n, int* info_h, int* info_d, hipStream_t s) {
+  RTA_HIP(hipMemsetAsync(info_d, 0, sizeof(int), s));
+  for (int64_t k0 = 0; k0 < n; k0 += CH_NB) {
+    const int kb = (int)std::min<int64_t>(CH_NB, n - k0);
+    hipLaunchKernelGGL(potf2_kernel, dim3(1), dim3(CH_NB), 0, s, W, n, k0, kb, info_d);
+    const int64_t r0 = k0 + kb;
+    if (r0 < n) {
+      hipLaunchKernelGGL(trsm_rows_kernel, dim3((unsigned)((n - r0 + CH_NB - 1) / CH_NB)),
+                         dim3(CH_NB), 0, s, W, n, k0, kb, r0, n);
+      const int64_t m2 = n - r0;
+      RTA_TRY(rt_dgemm_dev('N', 'T', m2, m2, kb, -1.0, W + r0 + n * k0, n, W + r0 + n * k0, n,
+                           1.0, W + r0 + n * r0, n, s));
+    }
+    RTA_HIP(hipGetLastError());
+  }
+  RTA_HIP(hipMemcpyAsync(info_h, info_d, sizeof(int), hipMemcpyDeviceToHost, s));
+  RTA_HIP(hipStreamSynchronize(s));
+  return RT_OK;
+}
+
+// X (ld n) <- L^-1 for the lower factor in Lf (ld n): block rows in order,
+// X[b,:] = L_bb^-1 (I_b - L[b, :k0] X[:k0, :])
+static int tri_inverse_lower(const double* Lf, int64_t n, double* X, hipStream_t s) {
+  hipLaunchKernelGGL(mt_fill_kernel, dim3(mt_grid(n * n)), dim3(256), 0, s, X, n * n, 0.0);
+  for (int64_t k0 = 0; k0 < n; k0 += CH_NB) {
+    const int kb = (int)std::min<int64_t>(CH_NB, n - k0);
+    const int64_t nc = k0 + kb;
+    hipLaunchKernelGGL(mt_eye_block_kernel, dim3(1), dim3(CH_NB), 0, s, X, n, k0, kb);
+    if (k0 > 0)
+      RTA_TRY(rt_dgemm_dev('N', 'N', kb, nc, k0, -1.0, Lf + k0, n, X, n, 1.0, X + k0, n, s));
+    hipLaunchKernelGGL(trsm_cols_kernel, dim3((unsigned)((nc + CH_NB - 1) / CH_NB)),
+                       dim3(CH_NB), 0, s, Lf, n, X, n, k0, kb, nc);
+    RTA_HIP(hipGetLastError());
+  }
+  return RT_OK;
+}
+
+struct MtBufs {
+  std::vector<double*> p;
+  int* info = nullptr;
+  ~MtBufs() {
+    for (double* q : p) (void)hipFree(q);
+    if (info) (void)hipFree(info);
+  }
+  int take(size_t count, double** out) {
+    double* q = nullptr;
+    if (hipMalloc(&q, sizeof(double) * std::max<size_t>(count, 1)) != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(RT_ERR_NOMEM, "metric: allocation of %zu doubles failed", count);
+    }
+    p.push_back(q);
+    *out = q;
+    return RT_OK;
+  }
+};
+
+// Sinv = S^-1 for the SPD matrix S (ld n) through its Cholesky factor
+static int spd_inverse(const double* S, int64_t n, double* Sinv, MtBufs& b, hipStream_t s,
+                       int* info_h) {
+  double *W, *X;
+  RTA_TRY(b.take((size_t)n * n, &W));
+  RTA_TRY(b.take((size_t)n * n, &X));
+  RTA_HIP(hipMemcpyAsync(W, S, sizeof(double) * n * n, hipMemcpyDeviceToDevice, s));
+  RTA_TRY(chol_lower(W, n, info_h, b.info, s));
+  if (*info_h != 0) return RT_OK;
+  RTA_TRY(tri_inverse_lower(W, n, X, s));
+  // S^-1 = L^-T L^-1 = X^T X
+  return rt_dgemm_dev('T', 'N', n, n, n, 1.0, X, n, X, n, 0.0, Sinv, n, s);
+}
+
+static bool is_upper(char uplo) { return uplo == 'U' || uplo == 'u'; }
+
+}  // namespace rta
+
+using namespace rta;
+
+extern "C" int rt_dpotrf_dev(char uplo, int64_t n, double* A, int64_t lda, int* info,
+                             void* stream) {
+  RTA_CHECK(uplo == 'U' || uplo == 'u' || uplo == 'L' || uplo == 'l',
+            "uplo must be 'U' or 'L'");
+  RTA_CHECK(n >= 0 && lda >= std::max<int64_t>(1, n), "bad dimensions");
+  RTA_CHECK(info, "info is NULL");
+  *info = 0;
+  if (n == 0) return RT_OK;
+  RTA_CHECK(A, "A is NULL");
+  hipStream_t s = (hipStream_t)stream;
+  MtBufs b;
+  RTA_HIP(hipMalloc(&b.info, sizeof(int)));
+  double* W;
+  RTA_TRY(b.take((size_t)n * n, &W));
+  KernelTimer kt("dpotrf", s);
+  hipLaunchKernelGGL(mt_symmetrize_kernel, dim3(mt_grid(n * n)), dim3(256), 0, s, A, lda, n,
+                     is_upper(uplo), W);
+  RTA_TRY(chol_lower(W, n, info, b.info, s));
+  // the factor replaces the uplo triangle; the other strict triangle is untouched
+  hipLaunchKernelGGL(mt_store_tri_kernel, dim3(mt_grid(n * n)), dim3(256), 0, s, W, n,
+                     is_upper(uplo), false, A, lda);
+  RTA_HIP(hipGetLastError());
+  RTA_HIP(hipStreamSynchronize(s));
+  return RT_OK;
+}
+
+extern "C" int rt_cholesky_decompose_inverse_dev(char uplo, int64_t n, const double* A,
+                                                 int64_t lda, double* out, int64_t ldo,
+                                                 void* stream) {
+  RTA_CHECK(uplo == 'U' || uplo == 'u' || uplo == 'L' || uplo == 'l',
+            "uplo must be 'U' or 'L'");
+  RTA_CHECK(n >= 0 && lda >= std::max<int64_t>(1, n) && ldo >= std::max<int64_t>(1, n),
+            "bad dimensions");
+  if (n == 0) return RT_OK;
+  RTA_CHECK(A && out, "NULL argument");
+  hipStream_t s = (hipStream_t)stream;
+  MtBufs b;
+  RTA_HIP(hipMalloc(&b.info, sizeof(int)));
+  double *S, *Sinv;
+  RTA_TRY(b.take((size_t)n * n, &S));
+  RTA_TRY(b.take((size_t)n * n, &Sinv));
+  KernelTimer kt("chol_inv", s);
+  hipLaunchKernelGGL(mt_symmetrize_kernel, dim3(mt_grid(n * n)), dim3(256), 0, s, A, lda, n,
+                     is_upper(uplo), S);
+  int info = 0;
+  RTA_TRY(spd_inverse(S, n, Sinv, b, s, &info));
+  if (info != 0)
+    return fail(RT_ERR_NUMERIC, "DPOTRF failed with %d (leading minor not positive definite)",
+                info);
+  // the factor of the inverse: S^-1 = M M^T (lower) or U^T U (upper: U = M^T);
+  // like the reference, a failure of this second factorization is not reported
+  RTA_TRY(chol_lower(Sinv, n, &info, b.info, s));
+  hipLaunchKernelGGL(mt_store_tri_kernel, dim3(mt_grid(n * n)), dim3(256), 0, s, Sinv, n,
+                     is_upper(uplo), true, out, ldo);
+  RTA_HIP(hipGetLastError());
+  RTA_HIP(hipStreamSynchronize(s));
+  return RT_OK;
+}
+
+extern "C" int rt_matrix_power_dev(int64_t n, const double* A, int64_t lda, double p,
+                                   double threshold, double* out, int64_t ldo, void* stream) {
+  RTA_CHECK(n >= 0 && lda >= std::max<int64_t>(1, n) && ldo >= std::max<int64_t>(1, n),
+            "bad dimensions");
+  RTA_CHECK(p == 1.0 || p == -1.0 || p == 0.5 || p == -0.5,
+            "power %g: only p = 1, -1, 1/2, -1/2 are built", p);
+  if (n == 0) return RT_OK;
+  RTA_CHECK(A && out, "NULL argument");
+  hipStream_t s = (hipStream_t)stream;
+  MtBufs b;
+  RTA_HIP(hipMalloc(&b.info, sizeof(int)));
+  double *S, *R;
+  RTA_TRY(b.take((size_t)n * n, &S));
+  RTA_TRY(b.take((size_t)n * n, &R));
+  KernelTimer kt("matrix_power", s);
+  hipLaunchKernelGGL(mt_symmetrize_kernel, dim3(mt_grid(n * n)), dim3(256), 0, s, A, lda, n,
+                     false, S);
+  // the reference drops eigenvalues below threshold; here none may be: check
+  // that S - threshold I is positive definite
+  int info = 0;
+  {
+    double* W;
+    RTA_TRY(b.take((size_t)n * n, &W));
+    hipLaunchKernelGGL(mt_axpi_kernel, dim3(mt_grid(n * n)), dim3(256), 0, s, S, n, 1.0,
+                       -std::max(threshold, 0.0), W);
+    RTA_TRY(chol_lower(W, n, &info, b.info, s));
+    if (info != 0)
+      return fail(RT_ERR_ARG,
+                  "matrix power: eigenvalues at or below the threshold %g (leading minor %d); "
+                  "the thresholded (pseudo) power is not built",
+                  threshold, info);
+  }
+  if (p == 1.0) {
+    RTA_HIP(hipMemcpyAsync(R, S, sizeof(double) * n * n, hipMemcpyDeviceToDevice, s));
+  } else if (p == -1.0) {
+    RTA_TRY(spd_inverse(S, n, R, b, s, &info));
+    if (info != 0) return fail(RT_ERR_NUMERIC, "matrix power: factorization failed (%d)", info);
+  } else {
+    // A = L L^T; the polar factor Q of L^T (L^T = Q H, H = A^(1/2)) by the
+    // Newton-Schulz iteration X <- X (3I - X^T X)/2 from X0 = L^T/|L|_F
+    // (singular values of L^T: the square roots of A's eigenvalues, so the
+    // iteration sees sqrt(cond A)); then A^(1/2) = Q^T L^T, A^(-1/2) = L^-T Q.
+    double *W, *Lt, *X, *G, *U, *sq;
+    RTA_TRY(b.take((size_t)n * n, &W));
+    RTA_TRY(b.take((size_t)n * n, &Lt));
+    RTA_TRY(b.take((size_t)n * n, &X));
+    RTA_TRY(b.take((size_t)n * n, &G));
+    RTA_TRY(b.take((size_t)n * n, &U));
+    RTA_TRY(b.take(512, &sq));
+    RTA_HIP(hipMemcpyAsync(W, S, sizeof(double) * n * n, hipMemcpyDeviceToDevice, s));
+    RTA_TRY(chol_lower(W, n, &info, b.info, s));
+    if (info != 0) return fail(RT_ERR_NUMERIC, "matrix power: factorization failed (%d)", info);
+    hipLaunchKernelGGL(mt_store_tri_kernel, dim3(mt_grid(n * n)), dim3(256), 0, s, W, n, true,
+                       true, Lt, n);  // Lt = L^T (upper), zeros below
+    RTA_TRY(dv_sqnorms(Lt, 1, n * n, sq, sq + 1, s));
+    double fro2 = 0.0;
+    RTA_HIP(hipMemcpyAsync(&fro2, sq, sizeof(double), hipMemcpyDeviceToHost, s));
+    RTA_HIP(hipStreamSynchronize(s));
+    hipLaunchKernelGGL(mt_axpi_kernel, dim3(mt_grid(n * n)), dim3(256), 0, s, Lt, n,
+                       1.0 / std::sqrt(fro2), 0.0, X);
+    double err = 1.0, prev = 1e300;
+    for (int it = 0; it < 200; ++it) {
+      RTA_TRY(rt_dgemm_dev('T', 'N', n, n, n, 1.0, X, n, X, n, 0.0, G, n, s));  // X^T X
+      hipLaunchKernelGGL(mt_axpi_kernel, dim3(mt_grid(n * n)), dim3(256), 0, s, G, n, -1.0,
+                         1.0, U);  // I - X^T X
+      RTA_TRY(dv_sqnorms(U, 1, n * n, sq, sq + 1, s));
+      RTA_HIP(hipMemcpyAsync(&err, sq, sizeof(double), hipMemcpyDeviceToHost, s));
+      RTA_HIP(hipStreamSynchronize(s));
+      err = std::sqrt(err);
+      // quadratic convergence down to rounding: stop at the floor
+      if (err < 1e-13 * std::sqrt((double)n) || (err < 1e-6 && err >= 0.5 * prev)) break;
+      prev = err;
+      hipLaunchKernelGGL(mt_axpi_kernel, dim3(mt_grid(n * n)), dim3(256), 0, s, G, n, -0.5,
+                         1.5, U);  // (3I - X^T X)/2
+      RTA_TRY(rt_dgemm_dev('N', 'N', n, n, n, 1.0, X, n, U, n, 0.0, G, n, s));
+      RTA_HIP(hipMemcpyAsync(X, G, sizeof(double) * n * n, hipMemcpyDeviceToDevice, s));
+    }
+    if (!(err < 1e-6))
+      return fail(RT_ERR_NUMERIC, "matrix power: polar iteration stalled at %.3e", err);
+    if (p == 0.5) {
+      RTA_TRY(rt_dgemm_dev('T', 'N', n, n, n, 1.0, X, n, Lt, n, 0.0, R, n, s));  // Q^T L^T
+    } else {
+      RTA_TRY(tri_inverse_lower(W, n, G, s));                                    // L^-1
+      RTA_TRY(rt_dgemm_dev('T', 'N', n, n, n, 1.0, G, n, X, n, 0.0, R, n, s));  // L^-T Q
+    }
+    hipLaunchKernelGGL(mt_sym_avg_kernel, dim3(mt_grid(n * n)), dim3(256), 0, s, R, n);
+  }
+  // out = R (symmetric)
+  RTA_HIP(hipMemcpy2DAsync(out, sizeof(double) * ldo, R, sizeof(double) * n,
+                           sizeof(double) * n, n, hipMemcpyDeviceToDevice, s));
+  RTA_HIP(hipGetLastError());
+  RTA_HIP(hipStreamSynchronize(s));
+  return RT_OK;
+}

@@ -226,3 +226,21 @@ def einsum_ip_ip_p(A: DeviceArray, a_rows, a_cols, B: DeviceArray, b_rows, b_col
                    out: DeviceArray, stream=None):
     check(lib().rt_einsum_ip_ip_p_dev(A.ptr, a_rows, a_cols, B.ptr, b_rows, b_cols,
                                       out.ptr, stream))
+
+
+# ---- FP64 aux-metric functions (A11) ---------------------------------------------
+def dpotrf(uplo, n, A: DeviceArray, lda, stream=None) -> int:
+    """Returns LAPACK's info (0 = success)."""
+    info = C.c_int(0)
+    check(lib().rt_dpotrf_dev(uplo.encode(), n, A.ptr, lda, C.byref(info), stream))
+    return info.value
+
+
+def cholesky_decompose_inverse(uplo, n, A: DeviceArray, lda, out: DeviceArray, ldo,
+                               stream=None):
+    check(lib().rt_cholesky_decompose_inverse_dev(uplo.encode(), n, A.ptr, lda, out.ptr,
+                                                  ldo, stream))
+
+
+def matrix_power(n, A: DeviceArray, lda, p, threshold, out: DeviceArray, ldo, stream=None):
+    check(lib().rt_matrix_power_dev(n, A.ptr, lda, p, threshold, out.ptr, ldo, stream))

@@ -349,3 +349,53 @@ def test_einsum_ip_ip_p_and_rij_composition():
     gamma_ref, J, e_j = O.rij(A3, Dm)
     assert np.allclose(gamma, gamma_ref, rtol=1e-13)
     assert np.allclose(O.upper_unpack(Jp), J, rtol=1e-13, atol=1e-13)
+
+
+# ---- FP64 aux-metric functions -------------------------------------------------
+def _lib_rs_8x8():
+    import json
+    import os
+    d = json.load(open(os.path.join(os.path.dirname(__file__), "golden",
+                                    "matfull_inverse_and_power_8x8.json")))
+    return np.array(d["data"]).reshape(d["size"], order="F")
+
+
+def test_dpotrf_doc_example():
+    from oracle import metric_oracle as M
+    up = np.array([4.0, 12.0, 37.0, -16.0, -43.0, 98.0])
+    A = O.upper_unpack(up)
+    c, info = M.dpotrf(A, "U")
+    assert info == 0
+    assert np.array_equal(O.upper_pack(c), [2.0, 6.0, 1.0, -8.0, 5.0, 3.0])
+
+
+def test_dsyev_doc_eigenvalues():
+    A = O.upper_unpack(np.arange(1.0, 7.0))
+    w = np.linalg.eigvalsh(A)
+    assert np.allclose(w, [-1.5066326307865059, -0.05739624271478554, 11.564028873501286],
+                       rtol=0, atol=1e-12)
+
+
+def test_metric_functions_on_reference_test_matrix():
+    from oracle import metric_oracle as M
+    S = _lib_rs_8x8()
+    assert np.allclose(S, S.T, atol=1e-15)
+    X = M.lapack_power(S, -0.5, 1e-5)
+    assert np.abs(X @ S @ X - np.eye(8)).max() < 1e-12
+    assert np.abs(M.lapack_power(S, 0.5, 1e-5) @ M.lapack_power(S, 0.5, 1e-5) - S).max() < 1e-12
+    for uplo in ("L", "U"):
+        Mf = M.cholesky_decompose_inverse(S, uplo)
+        F = Mf @ Mf.T if uplo == "L" else Mf.T @ Mf
+        assert np.abs(F - np.linalg.inv(S)).max() < 1e-11
+        assert np.array_equal(Mf, np.tril(Mf) if uplo == "L" else np.triu(Mf))
+
+
+def test_metric_closed_forms():
+    from oracle import metric_oracle as M
+    D = np.diag([4.0, 9.0, 0.25])
+    assert np.allclose(M.lapack_power(D, -0.5, 1e-10), np.diag([0.5, 1 / 3, 2.0]), rtol=1e-15)
+    assert np.allclose(M.lapack_power(np.diag([4.0, 1e-12]), -1.0, 1e-10),
+                       np.diag([0.25, 0.0]), rtol=1e-15)      # below threshold: dropped
+    Mf = M.cholesky_decompose_inverse(D, "L")
+    assert np.allclose(Mf, np.diag([0.5, 1 / 3, 2.0]), rtol=1e-15)
+    assert M.cholesky_decompose_inverse(np.diag([1.0, -1.0]), "L") is None
