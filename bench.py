@@ -56,7 +56,7 @@ def main():
                     help="molecules of the batched workload (0 = skip)")
     ap.add_argument("--davidson", type=int, default=1,
                     help="run the Davidson workload (plain vs learned preconditioner)")
-    ap.add_argument("--forms", default="ozaki_int8,ozaki_bf16,bf16x3",
+    ap.add_argument("--forms", default="ozaki_int8,split_fp32,ozaki_bf16,bf16x3",
                     help="comma-separated reduced-precision K forms to time ('' = none)")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="only run one plain device copy of 512 MiB (copy3d kernel: reads "
@@ -140,7 +140,8 @@ def main():
     K64 = dK.to_host((nmo, nmo))
     prec = {}
     dK2 = D.DeviceArray(nmo * nmo)
-    knames = {"ozaki_int8": "rik_oz_i8", "ozaki_bf16": "rik_ozaki", "bf16x3": "rik_bf16x3"}
+    knames = {"ozaki_int8": "rik_oz_i8", "split_fp32": "rik_split32",
+              "ozaki_bf16": "rik_ozaki", "bf16x3": "rik_bf16x3"}
     for form in [f for f in args.forms.split(",") if f]:
         kname = knames[form]
         for _ in range(max(1, args.warmup)):

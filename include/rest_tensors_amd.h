@@ -32,7 +32,8 @@ extern "C" {
 
 /* arithmetic of the GEMM-shaped stages (ao2mo, RI-K) */
 #define RT_PREC_FP64 0        /* FP64 MFMA                                 */
-#define RT_PREC_SPLIT_FP32 1  /* hi/lo FP32 split, 3 FP32-MFMA products    */
+#define RT_PREC_SPLIT_FP32 1  /* hi/lo FP32 split, 3 FP32-MFMA products,
+                                 FP64 fold per 32-deep k-chunk             */
 #define RT_PREC_BF16X3 2      /* hi/lo BF16 split, 3 BF16-MFMA products    */
 #define RT_PREC_OZAKI_BF16 3  /* exponent-aligned BF16 slices, exact FP32
                                  partial sums, FP64 accumulation           */

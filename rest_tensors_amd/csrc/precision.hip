@@ -1,10 +1,12 @@
 // Reduced-precision forms of the RI-K build.  RT_PREC_OZAKI_BF16 and
 // RT_PREC_BF16X3 slice B into BF16 planes (sliced.hpp) and run the K
 // contraction on BF16 MFMA with FP64 accumulation across aux slices;
-// RT_PREC_OZAKI_INT8 slices it into int8 planes for INT8 MFMA (ozaki_i8.hpp).
+// RT_PREC_OZAKI_INT8 slices it into int8 planes for INT8 MFMA (ozaki_i8.hpp),
+// RT_PREC_SPLIT_FP32 into hi/lo FP32 planes for FP32 MFMA (split32.hpp).
 #include "common.hpp"
 #include "ozaki_i8.hpp"
 #include "precision.hpp"
+#include "split32.hpp"
 #include "sliced.hpp"
 
 namespace rta {
@@ -18,6 +20,7 @@ int ao2mo_occ_lowp(const double*, int64_t, int64_t, int64_t, const double*,
 int rik_lowp(const double* B, int64_t nocc, int64_t nmo, int64_t naux,
              double* K, int precision, hipStream_t s) {
   if (precision == RT_PREC_OZAKI_INT8) return rik_ozaki_i8(B, nocc, nmo, naux, K, s);
+  if (precision == RT_PREC_SPLIT_FP32) return rik_split32(B, nocc, nmo, naux, K, s);
   int ns;
   bool aligned;
   switch (precision) {

@@ -84,3 +84,25 @@ def test_ozaki_int8_k(D, shape):
         Ke = O.rik_ozaki_i8(B, gp)
         assert relf(k8, Ke) < 1e-14
         assert relf(k8, O.rik(B)[0]) < 1e-8
+
+
+@pytest.mark.parametrize("shape", [(128, 384, 16), (96, 50, 5), (200, 300, 70),
+                                   (512, 2048, 64)])
+def test_split_fp32_k(D, shape):
+    """Split FP32 (hi/lo planes, 3 FP32-MFMA products, FP64 fold per 32-deep
+    k-chunk): the FP32 rounding of each chunk's hi*hi sum bounds the
+    off-diagonal K entries near the FP32 unit roundoff, so the stated
+    tolerance is 5e-8 for K (relative Frobenius) and 1e-8 for E_x (a sum of
+    diagonal entries); exactly symmetric."""
+    nbas, naux, nocc = shape
+    print(f"split_fp32 shape {shape}", flush=True)
+    dB = make_B(D, nbas, naux, nocc, seed=5000 + nbas)
+    K64, K32 = D.DeviceArray(nbas * nbas), D.DeviceArray(nbas * nbas)
+    ex64 = D.rik(dB, nocc, nbas, naux, K64, precision="fp64")
+    ex32 = D.rik(dB, nocc, nbas, naux, K32, precision="split_fp32")
+    k64, k32 = K64.to_host((nbas, nbas)), K32.to_host((nbas, nbas))
+    err = relf(k32, k64)
+    print(f"{shape}: K rel err split_fp32 {err:.2e}, E_x rel err "
+          f"{abs(ex32 - ex64) / abs(ex64):.2e}")
+    assert err < 5e-8 and abs(ex32 - ex64) < 1e-8 * abs(ex64)
+    assert np.array_equal(k32, k32.T)
