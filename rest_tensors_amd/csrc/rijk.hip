@@ -27,32 +27,46 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-// One block per aux slice P; wave w walks columns j = w, w+4, ...; lanes walk
-// rows 0..j of that column.  Fixed order -> deterministic.
+// One block per pair of aux slices (P, P+1); wave w walks columns j = w,
+// w+4, ...; lanes walk rows of that column.  Each D element read serves both
+// slices.  Fixed order -> deterministic.
 __global__ __launch_bounds__(256) void rij_gamma_kernel(
     const double* __restrict__ A, const double* __restrict__ D, int64_t nbas,
-    double* __restrict__ gamma) {
-  __shared__ double red[4];
-  const int64_t P = blockIdx.x;
-  const double* Ap = A + P * nbas * nbas;
+    int64_t naux, double* __restrict__ gamma) {
+  __shared__ double red[2][4];
+  const int64_t P = 2 * (int64_t)blockIdx.x;
+  const int64_t n2 = nbas * nbas;
+  const bool two = P + 1 < naux;
+  const double* Ap = A + P * n2;
+  const double* Aq = two ? Ap + n2 : Ap;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  double acc = 0.0;
+  double acc = 0.0, acc2 = 0.0;
   for (int64_t j = wave; j < nbas; j += 4) {
-    const double* col = Ap + j * nbas;
+    const double* cp = Ap + j * nbas;
+    const double* cq = Aq + j * nbas;
     const double* dcol = D + j * nbas;
     for (int64_t i = lane; i <= j; i += 64) {
-      double w = (i == j) ? 1.0 : 2.0;
-      acc += w * col[i] * dcol[i];
+      const double dw = (i == j) ? dcol[i] : 2.0 * dcol[i];
+      acc += cp[i] * dw;
+      acc2 += cq[i] * dw;
     }
   }
   acc = wave_sum(acc);
-  if (lane == 0) red[wave] = acc;
+  acc2 = wave_sum(acc2);
+  if (lane == 0) {
+    red[0][wave] = acc;
+    red[1][wave] = acc2;
+  }
   __syncthreads();
-  if (threadIdx.x == 0) gamma[P] = (red[0] + red[1]) + (red[2] + red[3]);
+  if (threadIdx.x == 0) gamma[P] = (red[0][0] + red[0][1]) + (red[0][2] + red[0][3]);
+  if (threadIdx.x == 1 && two)
+    gamma[P + 1] = (red[1][0] + red[1][1]) + (red[1][2] + red[1][3]);
 }
 
 // Block = (column j, 64-row segment s of rows 0..j); its 4 waves take every
-// 4th aux slice and are combined through LDS.  Writes J[i,j] and J[j,i].
+// 4th aux slice (8 loads in flight per lane; the wave-uniform gamma index
+// lets gamma come through scalar loads) and are combined through LDS.
+// Writes J[i,j] and J[j,i].
 __global__ __launch_bounds__(256) void rij_j_kernel(
     const double* __restrict__ A, const double* __restrict__ gamma,
     int64_t nbas, int64_t naux, double* __restrict__ J) {
@@ -60,24 +74,28 @@ __global__ __launch_bounds__(256) void rij_j_kernel(
   const int64_t j = blockIdx.x;
   const int64_t i0 = (int64_t)blockIdx.y * 64;
   if (i0 > j) return;  // segment entirely below the diagonal
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const double* gm = gamma;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int64_t i = i0 + lane;
   const int64_t n2 = nbas * nbas;
-  double acc = 0.0;
+  double acc = 0.0, acc2 = 0.0;
   if (i <= j) {
     const double* p = A + i + j * nbas;
     int64_t P = wave;
-    for (; P + 12 < naux; P += 16) {
-      double a0 = p[P * n2], a1 = p[(P + 4) * n2], a2 = p[(P + 8) * n2],
-             a3 = p[(P + 12) * n2];
-      acc += a0 * gamma[P];
-      acc += a1 * gamma[P + 4];
-      acc += a2 * gamma[P + 8];
-      acc += a3 * gamma[P + 12];
+    for (; P + 28 < naux; P += 32) {
+      double a[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) a[u] = p[(P + 4 * u) * n2];
+#pragma unroll
+      for (int u = 0; u < 8; u += 2) {
+        acc += a[u] * gm[P + 4 * u];
+        acc2 += a[u + 1] * gm[P + 4 * u + 4];
+      }
     }
-    for (; P < naux; P += 4) acc += p[P * n2] * gamma[P];
+    for (; P < naux; P += 4) acc += p[P * n2] * gm[P];
   }
-  red[wave][lane] = acc;
+  red[wave][lane] = acc + acc2;
   __syncthreads();
   if (wave == 0 && i <= j) {
     double v = (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
@@ -162,8 +180,8 @@ extern "C" int rt_rij_dev(const double* ri3ao, int64_t nbas, int64_t naux,
     RTA_TRY(scratch_get(SCRATCH_JK, sizeof(double) * naux, (void**)&g));
   if (naux > 0) {
     KernelTimer kt("rij_gamma", s);
-    hipLaunchKernelGGL(rij_gamma_kernel, dim3((unsigned)naux), dim3(256), 0, s,
-                       ri3ao, D, nbas, g);
+    hipLaunchKernelGGL(rij_gamma_kernel, dim3((unsigned)((naux + 1) / 2)), dim3(256), 0, s,
+                       ri3ao, D, nbas, naux, g);
     RTA_HIP(hipGetLastError());
   }
   {
