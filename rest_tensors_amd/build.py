@@ -26,6 +26,15 @@ COMMON = [
 ]
 
 
+# Per-file flags.  The FP64 GEMM engines keep their 128 accumulator registers
+# in VGPRs: with the default MFMA form the compiler holds them in AGPRs only
+# around the MFMAs and copies all of them in and out every k-chunk.
+FILE_FLAGS = {
+    "gemm.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form=1"],
+    "batch.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form=1"],
+}
+
+
 def sources():
     out = []
     for f in sorted(os.listdir(CSRC)):
@@ -36,7 +45,8 @@ def sources():
 
 def _compile(src: str) -> str:
     obj = os.path.join(OBJ, os.path.basename(src) + ".o")
-    cmd = [HIPCC, *COMMON, "-x", "hip", "-c", src, "-o", obj]
+    cmd = [HIPCC, *COMMON, *FILE_FLAGS.get(os.path.basename(src), []), "-x", "hip", "-c",
+           src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")

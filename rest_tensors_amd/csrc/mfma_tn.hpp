@@ -124,18 +124,26 @@ __global__ __launch_bounds__(256) void dgemm_tn_kernel(TnParams g) {
 #pragma unroll
     for (int j = 0; j < FN; ++j) acc[i][j] = f64x4{0.0, 0.0, 0.0, 0.0};
 
-  auto issue = [&](int64_t it) {
-    const int64_t b = b0 + it / nk;
-    const int64_t k0 = (it % nk) * BK;
-    double* st = smem + (it % NSTAGE) * STAGE;
-    tn_issue<BM, BK>(g.A + b * g.sA, g.lda, m0, g.M, k0, st);
-    tn_issue<BN, BK>(g.B + b * g.sB, g.ldb, n0, g.N, k0, st + BM * BK);
+  // next chunk to load: batch ib, k offset ik, LDS stage islot (counters
+  // advanced per issue; no division in the loop)
+  int64_t ib = b0, ik = 0;
+  int islot = 0;
+  auto issue = [&]() {
+    double* st = smem + islot * STAGE;
+    tn_issue<BM, BK>(g.A + ib * g.sA, g.lda, m0, g.M, ik, st);
+    tn_issue<BN, BK>(g.B + ib * g.sB, g.ldb, n0, g.N, ik, st + BM * BK);
+    ik += BK;
+    if (ik == nk * BK) {
+      ik = 0;
+      ++ib;
+    }
+    islot = (islot + 1 == NSTAGE) ? 0 : islot + 1;
   };
 
   // prologue: NSTAGE-1 chunks in flight
 #pragma unroll
   for (int s = 0; s < NSTAGE - 1; ++s)
-    if (s < niter) issue(s);
+    if (s < niter) issue();
 
   constexpr int PER_STAGE = (BM + BN) * BK / 512;  // load instructions per wave
   for (int64_t it = 0; it < niter; ++it) {
@@ -150,32 +158,41 @@ __global__ __launch_bounds__(256) void dgemm_tn_kernel(TnParams g) {
     // reads completed before its MFMAs).  A bare s_barrier: __syncthreads()
     // would also drain vmcnt and serialize the load pipeline.
     __builtin_amdgcn_s_barrier();
-    if (it + NSTAGE - 1 < niter) issue(it + NSTAGE - 1);
+    if (it + NSTAGE - 1 < niter) issue();  // chunk it + NSTAGE - 1
 
-    const double* As = smem + (it % NSTAGE) * STAGE;
+    const double* As = smem + (int)(it % NSTAGE) * STAGE;
     const double* Bs = As + BM * BK;
-#pragma unroll
-    for (int kp = 0; kp < BK / 8; ++kp) {
+    // fragments of k-pair group kp (16 B per lane per 16-row block); two sets
+    // in flight: the reads of group kp+1 are issued before the MFMAs of kp,
+    // and a scheduling barrier keeps the compiler from hoisting every group's
+    // reads (which would need 128 VGPRs and push the accumulators out)
+    f64x2 a[2][FM], b[2][FN];
+    auto load = [&](int kp, int set) {
       const int c = kp * 4 + (lane >> 4);
-      f64x2 a[FM], b[FN];
 #pragma unroll
       for (int i = 0; i < FM; ++i) {
         const int r = wave_m * WM + i * 16 + (lane & 15);
-        a[i] = *(const f64x2*)(As + r * BK + 2 * (c ^ tn_swz<BK>(r)));
+        a[set][i] = *(const f64x2*)(As + r * BK + 2 * (c ^ tn_swz<BK>(r)));
       }
 #pragma unroll
       for (int j = 0; j < FN; ++j) {
         const int r = wave_n * WN + j * 16 + (lane & 15);
-        b[j] = *(const f64x2*)(Bs + r * BK + 2 * (c ^ tn_swz<BK>(r)));
+        b[set][j] = *(const f64x2*)(Bs + r * BK + 2 * (c ^ tn_swz<BK>(r)));
       }
+    };
+    load(0, 0);
+#pragma unroll
+    for (int kp = 0; kp < BK / 8; ++kp) {
+      if (kp + 1 < BK / 8) load(kp + 1, (kp + 1) & 1);
 #pragma unroll
       for (int s = 0; s < 2; ++s)
 #pragma unroll
         for (int i = 0; i < FM; ++i)
 #pragma unroll
           for (int j = 0; j < FN; ++j)
-            acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[j][s], a[i][s],
+            acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[kp & 1][j][s], a[kp & 1][i][s],
                                                              acc[i][j], 0, 0, 0);
+      __builtin_amdgcn_sched_barrier(0);
     }
   }
 
