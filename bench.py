@@ -261,7 +261,7 @@ def main():
                      "traffic": MEASURED_TRAFFIC_RIK.get((nbas, naux, nocc)),
                      "traffic_unit": "bytes per launch",
                      "algorithmic_bytes": 8.0 * nocc * nmo * naux,
-                     "kernel": "dgemm_tn_kernel<128,128,32,2> (rik_mfma)",
+                     "kernel": "dgemm_tn_kernel<128,128,16,2> (rik_mfma)",
                      "kernel_avg_ms": round(k_avg_ms, 4)},
         "cpu_baseline": cpu,
         "extras": extras,

@@ -60,7 +60,10 @@ int device_cu_count() {
 }
 
 // Engine configuration knobs (environment, read once): RTA_TN_STAGES in
-// {2,3,4} (k-chunks in flight), RTA_TN_BK in {16,32} (k-chunk depth).
+// {2,3,4} (k-chunks in flight), RTA_TN_BK in {16,32} (k-chunk depth; unset:
+// 16 for 128-row tiles, 32 for 64-row tiles).  The 128-row default, BK 16 with
+// 2 stages, takes 64 KB of LDS so two workgroups share a CU and one's barrier
+// waits hide behind the other's MFMAs (228 VGPRs: two waves per SIMD).
 static int env_int(const char* name, int def) {
   const char* e = getenv(name);
   return e ? atoi(e) : def;
@@ -74,8 +77,8 @@ static int tn_stages() {
 }
 static int tn_bk_pref() {
   static int bk = [] {
-    int v = env_int("RTA_TN_BK", 32);
-    return (v == 16 || v == 32) ? v : 32;
+    int v = env_int("RTA_TN_BK", 0);
+    return (v == 16 || v == 32) ? v : 0;
   }();
   return bk;
 }
@@ -132,7 +135,8 @@ static int tn_launch(const GemmParams& g, int tile, hipStream_t s,
   p.reduce = g.reduce;
   p.upper = g.upper;
   KernelTimer kt(timer_name, s);
-  const bool bk32 = tn_bk_pref() == 32 && p.K % 32 == 0;
+  const int bk = tn_bk_pref() ? tn_bk_pref() : (tile == 128 ? 16 : 32);
+  const bool bk32 = bk == 32 && p.K % 32 == 0;
   if (tile == 128) {
     if (bk32) return tn_launch_cfg<128, 128, 32, 2>(p, s);  // 2 x 64 KB LDS
     switch (tn_stages()) {

@@ -118,6 +118,11 @@ __global__ __launch_bounds__(256) void dgemm_tn_kernel(TnParams g) {
   const int wave = threadIdx.x >> 6;
   const int wave_m = wave & 1, wave_n = wave >> 1;
 
+  // Upper mode: a wave tile strictly below the diagonal is never read.  Its
+  // wave keeps loading and synchronising but issues no LDS reads or MFMAs,
+  // which leaves its SIMD's MFMA pipe to the co-resident workgroup.
+  const bool idle = g.upper && (m0 + wave_m * WM >= n0 + wave_n * WN + WN);
+
   f64x4 acc[FM][FN];
 #pragma unroll
   for (int i = 0; i < FM; ++i)
@@ -159,6 +164,7 @@ __global__ __launch_bounds__(256) void dgemm_tn_kernel(TnParams g) {
     // would also drain vmcnt and serialize the load pipeline.
     __builtin_amdgcn_s_barrier();
     if (it + NSTAGE - 1 < niter) issue();  // chunk it + NSTAGE - 1
+    if (idle) continue;
 
     const double* As = smem + (int)(it % NSTAGE) * STAGE;
     const double* Bs = As + BM * BK;
@@ -197,6 +203,7 @@ __global__ __launch_bounds__(256) void dgemm_tn_kernel(TnParams g) {
   }
 
   // epilogue: lane l owns row m = .. + l%16 of columns n = .. + l/16 + 4*r
+  if (idle) return;
   double* out = g.C + chunk * g.sC;
 #pragma unroll
   for (int i = 0; i < FM; ++i) {
