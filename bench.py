@@ -28,11 +28,11 @@ PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
 
 # HBM bytes per launch of the FP64 RI-K kernel, measured with rocprofv3 --pmc
-# (profiles/ri_pipeline_nbas1024_naux4096_nocc128.md): FETCH_SIZE 3473562 KiB
-# x 2.000 (a plain 512 MiB copy reads as 262182 KiB) + WRITE_SIZE 294912 KiB
-# x 1.000 (exact on the same copy).  Keyed by (nbas, naux, nocc); other shapes
-# report null.
-MEASURED_TRAFFIC_RIK = {(1024, 4096, 128): (3473562 * 2.000 + 294912 * 1.000) * 1024}
+# (profiles/ri_pipeline_nbas1024_naux4096_nocc128.md): FETCH_SIZE 5181450 KiB
+# (mean of two dispatches) x 2.000 (a plain 512 MiB copy reads as 262182 KiB)
+# + WRITE_SIZE 278528 KiB x 1.000 (exact on the same copy).  Keyed by
+# (nbas, naux, nocc); other shapes report null.
+MEASURED_TRAFFIC_RIK = {(1024, 4096, 128): (5181450 * 2.000 + 278528 * 1.000) * 1024}
 
 
 def log(*a):
