@@ -191,14 +191,29 @@ int rt_dpotrf_dev(char uplo, int64_t n, double* A, int64_t lda, int* info,
 int rt_cholesky_decompose_inverse_dev(char uplo, int64_t n, const double* A,
                                       int64_t lda, double* out, int64_t ldo,
                                       void* stream);
-/* out = A^p for symmetric A (lower triangle read) and p in {1, -1, 1/2, -1/2}
- * (inverse through Cholesky; square roots through the polar factor of the
- * Cholesky factor, Newton-Schulz on FP64 GEMMs).  The reference (MatrixFull::lapack_power,
- * src/matrix/matrix_blas_lapack.rs:1004-1064) drops eigenvalues below
- * `threshold`; here every eigenvalue must lie above it (checked by factoring
- * A - threshold I), else RT_ERR_ARG.  Synchronizes. */
+/* out = A^p in the reference's sense (MatrixFull::lapack_power,
+ * src/matrix/matrix_blas_lapack.rs:1004-1064; _power, :599-653): the lower
+ * triangle of A is eigen-decomposed (rt_dsyev_dev on -A, so eigenvalues come
+ * largest first), eigenvalues below `threshold` are dropped, and
+ * out = sum_{lambda >= threshold} sqrt(lambda)^p sqrt(lambda)^p u u^T.
+ * Any real p.  Synchronizes. */
 int rt_matrix_power_dev(int64_t n, const double* A, int64_t lda, double p,
                         double threshold, double* out, int64_t ldo, void* stream);
+/* All eigenvalues (ascending, w: n doubles on the device) and, for
+ * jobz 'V', the orthonormal eigenvectors (overwriting A's columns) of the
+ * symmetric matrix whose lower triangle A holds: LAPACK DSYEV semantics with
+ * uplo 'L', as _dsyev calls it (src/matrix/matrix_blas_lapack.rs:319-348).
+ * info as LAPACK: 0, or the number of off-diagonal entries of the
+ * tridiagonal form that did not converge (the reference panics on it).
+ * Tridiagonal reduction, rotation application and back-transformation run
+ * on the device; the QL/QR iteration on the tridiagonal runs on the host.
+ * Synchronizes. */
+int rt_dsyev_dev(char jobz, int64_t n, double* A, int64_t lda, double* w, int* info,
+                 void* stream);
+/* Host-only: LAPACK DSTEQR on host arrays (compz 'N', 'I' or 'V'; d[n],
+ * e[n-1], z[n x n] with ld ldz), the tridiagonal stage of rt_dsyev_dev. */
+int rt_dsteqr_host(char compz, int64_t n, double* d, double* e, double* z, int64_t ldz,
+                   int* info);
 
 /* ---- packed-upper MatrixUpper (A8/A9) ---------------------------------- */
 /* Packed index of (i,j), i<=j: j*(j+1)/2 + i (index.rs:209-233).

@@ -4,15 +4,20 @@ Restates, with scipy's LAPACK (the library the reference calls through the
 lapack crate), the sequences of
   _dpotrf                     src/matrix/matrix_blas_lapack.rs:518-534
   cholesky_decompose_inverse  src/matrix/matrix_blas_lapack.rs:931-1002
+  _dsyev                      src/matrix/matrix_blas_lapack.rs:319-348 (DSYEV, uplo 'L')
   lapack_power                src/matrix/matrix_blas_lapack.rs:1004-1064
-The product never imports this module.
+and DSTEQR with vectors from the identity (through DSTEV), the tridiagonal
+stage of DSYEV.  The product never imports this module.
 
 Pinning: dpotrf by the doc example of matrix_blas_lapack.rs:502-517
-(U = [2,6,1,-8,5,3]); the eigen-decomposition step of lapack_power by the
-_dsyev doc eigenvalues (matrix_blas_lapack.rs:286-318); lapack_power(-0.5) by
-X S X = I and cholesky_decompose_inverse by M M^T = S^-1 on the 8x8 matrix of
-the reference's own test (src/lib.rs:170-190), and both against the closed
-forms on diagonal and 2x2 matrices.
+(U = [2,6,1,-8,5,3]); dsyev by the _dsyev doc example (eigenvalues and the
+sign-sensitive eigenvectors, matrix_blas_lapack.rs:286-318); dstev by the
+closed form of the [-1, 2, -1] tridiagonal (eigenvalues 2 - 2 cos(k pi/(n+1)),
+eigenvectors sin(j k pi/(n+1)) up to sign); lapack_power(-0.5) by X S X = I
+and cholesky_decompose_inverse by M M^T = S^-1 on the 8x8 matrix of the
+reference's own test (src/lib.rs:170-190), and both against the closed forms
+on diagonal and 2x2 matrices; the threshold drop of lapack_power against a
+matrix with known eigenpairs.
 """
 from __future__ import annotations
 
@@ -42,14 +47,35 @@ def cholesky_decompose_inverse(A: np.ndarray, uplo: str):
     return np.tril(m) if lower else np.triu(m)
 
 
+def dsyev(A: np.ndarray, jobz: str = "V"):
+    """LAPACK dsyev on the lower triangle (what _dsyev calls).  Returns
+    (eigenvectors or None, ascending eigenvalues, info)."""
+    vec = jobz.upper() == "V"
+    w, v, info = lapack.dsyev(np.asfortranarray(A), compute_v=1 if vec else 0, lower=1)
+    return (v if vec else None), w, int(info)
+
+
+def dstev(d: np.ndarray, e: np.ndarray):
+    """LAPACK dstev with vectors = DSTEQR from the identity on the
+    tridiagonal (d, e).  Returns (eigenvalues, eigenvectors, info)."""
+    d = np.array(d, float)
+    if d.shape[0] == 1:                  # scipy's wrapper wants a non-empty e
+        return d.copy(), np.ones((1, 1)), 0
+    w, z, info = lapack.dstev(d, np.array(e, float), compute_v=1)
+    return w, z, int(info)
+
+
 def lapack_power(A: np.ndarray, p: float, threshold: float):
     """Eigen-decomposition of -A (dsyev, ascending), so eigenvalues of A come
     largest first; those >= threshold are kept, their eigenvectors scaled by
     sqrt(lambda)^p, and the result is E E^T (the others are dropped)."""
-    w, v = np.linalg.eigh(-np.asarray(A))
+    v, w, info = dsyev(-np.asarray(A), "V")
+    assert info == 0
     w = -w
+    f = np.zeros_like(w)
     keep = w >= threshold
-    E = v[:, keep] * (np.sqrt(w[keep]) ** p)[None, :]
+    f[keep] = np.sqrt(w[keep]) ** p
+    E = v * f[None, :]
     return E @ E.T
 
 

@@ -1,13 +1,11 @@
 // FP64 aux-metric functions kept in FP64 on the RI path (A11):
 //   _dpotrf                     src/matrix/matrix_blas_lapack.rs:518-534
 //   cholesky_decompose_inverse  src/matrix/matrix_blas_lapack.rs:931-1002
-//   lapack_power (p = 1, -1, +-1/2)  src/matrix/matrix_blas_lapack.rs:1004-1064
 // Blocked right-looking Cholesky with 64-wide panels: the diagonal block is
 // factored by one workgroup in LDS, the panel below by a row-parallel
 // triangular solve, the trailing matrix by the FP64 MFMA GEMM.  Inverses go
 // through the triangular inverse (blocked forward substitution) and one TN
-// GEMM; square roots through the polar factor of the Cholesky factor
-// (Newton-Schulz on GEMMs; stable, and it sees only sqrt(cond A)).
+// GEMM.  (lapack_power, an eigen-decomposition, lives with the eigensolver.)
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -15,7 +13,6 @@
 #include <vector>
 
 #include "common.hpp"
-#include "dvkernels.hpp"
 #include "timer.hpp"
 
 namespace rta {
@@ -139,25 +136,6 @@ __global__ void mt_fill_kernel(double* __restrict__ X, int64_t len, double v) {
 __global__ void mt_eye_block_kernel(double* __restrict__ X, int64_t n, int64_t k0, int kb) {
   const int t = threadIdx.x;
   if (t < kb) X[(k0 + t) + n * (k0 + t)] = 1.0;
-}
-
-// M <- (M + M^T)/2 in place (each pair handled by the thread of its lower element)
-__global__ void mt_sym_avg_kernel(double* __restrict__ M, int64_t n) {
-  const int64_t e = blockIdx.x * (int64_t)256 + threadIdx.x;
-  if (e >= n * n) return;
-  const int64_t i = e % n, j = e / n;
-  if (i <= j) return;
-  const double v = 0.5 * (M[i + n * j] + M[j + n * i]);
-  M[i + n * j] = v;
-  M[j + n * i] = v;
-}
-
-// B = a * A + b * I (n x n)
-__global__ void mt_axpi_kernel(const double* __restrict__ A, int64_t n, double a, double b,
-                               double* __restrict__ B) {
-  const int64_t e = blockIdx.x * (int64_t)256 + threadIdx.x;
-  if (e >= n * n) return;
-  B[e] = a * A[e] + ((e % n == e / n) ? b : 0.0);
 }
 
 // W (ld n) <- lower Cholesky factor of W in its lower triangle (the strict
@@ -293,101 +271,6 @@ extern "C" int rt_cholesky_decompose_inverse_dev(char uplo, int64_t n, const dou
   RTA_TRY(chol_lower(Sinv, n, &info, b.info, s));
   hipLaunchKernelGGL(mt_store_tri_kernel, dim3(mt_grid(n * n)), dim3(256), 0, s, Sinv, n,
                      is_upper(uplo), true, out, ldo);
-  RTA_HIP(hipGetLastError());
-  RTA_HIP(hipStreamSynchronize(s));
-  return RT_OK;
-}
-
-extern "C" int rt_matrix_power_dev(int64_t n, const double* A, int64_t lda, double p,
-                                   double threshold, double* out, int64_t ldo, void* stream) {
-  RTA_CHECK(n >= 0 && lda >= std::max<int64_t>(1, n) && ldo >= std::max<int64_t>(1, n),
-            "bad dimensions");
-  RTA_CHECK(p == 1.0 || p == -1.0 || p == 0.5 || p == -0.5,
-            "power %g: only p = 1, -1, 1/2, -1/2 are built", p);
-  if (n == 0) return RT_OK;
-  RTA_CHECK(A && out, "NULL argument");
-  hipStream_t s = (hipStream_t)stream;
-  MtBufs b;
-  RTA_HIP(hipMalloc(&b.info, sizeof(int)));
-  double *S, *R;
-  RTA_TRY(b.take((size_t)n * n, &S));
-  RTA_TRY(b.take((size_t)n * n, &R));
-  KernelTimer kt("matrix_power", s);
-  hipLaunchKernelGGL(mt_symmetrize_kernel, dim3(mt_grid(n * n)), dim3(256), 0, s, A, lda, n,
-                     false, S);
-  // the reference drops eigenvalues below threshold; here none may be: check
-  // that S - threshold I is positive definite
-  int info = 0;
-  {
-    double* W;
-    RTA_TRY(b.take((size_t)n * n, &W));
-    hipLaunchKernelGGL(mt_axpi_kernel, dim3(mt_grid(n * n)), dim3(256), 0, s, S, n, 1.0,
-                       -std::max(threshold, 0.0), W);
-    RTA_TRY(chol_lower(W, n, &info, b.info, s));
-    if (info != 0)
-      return fail(RT_ERR_ARG,
-                  "matrix power: eigenvalues at or below the threshold %g (leading minor %d); "
-                  "the thresholded (pseudo) power is not built",
-                  threshold, info);
-  }
-  if (p == 1.0) {
-    RTA_HIP(hipMemcpyAsync(R, S, sizeof(double) * n * n, hipMemcpyDeviceToDevice, s));
-  } else if (p == -1.0) {
-    RTA_TRY(spd_inverse(S, n, R, b, s, &info));
-    if (info != 0) return fail(RT_ERR_NUMERIC, "matrix power: factorization failed (%d)", info);
-  } else {
-    // A = L L^T; the polar factor Q of L^T (L^T = Q H, H = A^(1/2)) by the
-    // Newton-Schulz iteration X <- X (3I - X^T X)/2 from X0 = L^T/|L|_F
-    // (singular values of L^T: the square roots of A's eigenvalues, so the
-    // iteration sees sqrt(cond A)); then A^(1/2) = Q^T L^T, A^(-1/2) = L^-T Q.
-    double *W, *Lt, *X, *G, *U, *sq;
-    RTA_TRY(b.take((size_t)n * n, &W));
-    RTA_TRY(b.take((size_t)n * n, &Lt));
-    RTA_TRY(b.take((size_t)n * n, &X));
-    RTA_TRY(b.take((size_t)n * n, &G));
-    RTA_TRY(b.take((size_t)n * n, &U));
-    RTA_TRY(b.take(512, &sq));
-    RTA_HIP(hipMemcpyAsync(W, S, sizeof(double) * n * n, hipMemcpyDeviceToDevice, s));
-    RTA_TRY(chol_lower(W, n, &info, b.info, s));
-    if (info != 0) return fail(RT_ERR_NUMERIC, "matrix power: factorization failed (%d)", info);
-    hipLaunchKernelGGL(mt_store_tri_kernel, dim3(mt_grid(n * n)), dim3(256), 0, s, W, n, true,
-                       true, Lt, n);  // Lt = L^T (upper), zeros below
-    RTA_TRY(dv_sqnorms(Lt, 1, n * n, sq, sq + 1, s));
-    double fro2 = 0.0;
-    RTA_HIP(hipMemcpyAsync(&fro2, sq, sizeof(double), hipMemcpyDeviceToHost, s));
-    RTA_HIP(hipStreamSynchronize(s));
-    hipLaunchKernelGGL(mt_axpi_kernel, dim3(mt_grid(n * n)), dim3(256), 0, s, Lt, n,
-                       1.0 / std::sqrt(fro2), 0.0, X);
-    double err = 1.0, prev = 1e300;
-    for (int it = 0; it < 200; ++it) {
-      RTA_TRY(rt_dgemm_dev('T', 'N', n, n, n, 1.0, X, n, X, n, 0.0, G, n, s));  // X^T X
-      hipLaunchKernelGGL(mt_axpi_kernel, dim3(mt_grid(n * n)), dim3(256), 0, s, G, n, -1.0,
-                         1.0, U);  // I - X^T X
-      RTA_TRY(dv_sqnorms(U, 1, n * n, sq, sq + 1, s));
-      RTA_HIP(hipMemcpyAsync(&err, sq, sizeof(double), hipMemcpyDeviceToHost, s));
-      RTA_HIP(hipStreamSynchronize(s));
-      err = std::sqrt(err);
-      // quadratic convergence down to rounding: stop at the floor
-      if (err < 1e-13 * std::sqrt((double)n) || (err < 1e-6 && err >= 0.5 * prev)) break;
-      prev = err;
-      hipLaunchKernelGGL(mt_axpi_kernel, dim3(mt_grid(n * n)), dim3(256), 0, s, G, n, -0.5,
-                         1.5, U);  // (3I - X^T X)/2
-      RTA_TRY(rt_dgemm_dev('N', 'N', n, n, n, 1.0, X, n, U, n, 0.0, G, n, s));
-      RTA_HIP(hipMemcpyAsync(X, G, sizeof(double) * n * n, hipMemcpyDeviceToDevice, s));
-    }
-    if (!(err < 1e-6))
-      return fail(RT_ERR_NUMERIC, "matrix power: polar iteration stalled at %.3e", err);
-    if (p == 0.5) {
-      RTA_TRY(rt_dgemm_dev('T', 'N', n, n, n, 1.0, X, n, Lt, n, 0.0, R, n, s));  // Q^T L^T
-    } else {
-      RTA_TRY(tri_inverse_lower(W, n, G, s));                                    // L^-1
-      RTA_TRY(rt_dgemm_dev('T', 'N', n, n, n, 1.0, G, n, X, n, 0.0, R, n, s));  // L^-T Q
-    }
-    hipLaunchKernelGGL(mt_sym_avg_kernel, dim3(mt_grid(n * n)), dim3(256), 0, s, R, n);
-  }
-  // out = R (symmetric)
-  RTA_HIP(hipMemcpy2DAsync(out, sizeof(double) * ldo, R, sizeof(double) * n,
-                           sizeof(double) * n, n, hipMemcpyDeviceToDevice, s));
   RTA_HIP(hipGetLastError());
   RTA_HIP(hipStreamSynchronize(s));
   return RT_OK;

@@ -244,3 +244,23 @@ def cholesky_decompose_inverse(uplo, n, A: DeviceArray, lda, out: DeviceArray, l
 
 def matrix_power(n, A: DeviceArray, lda, p, threshold, out: DeviceArray, ldo, stream=None):
     check(lib().rt_matrix_power_dev(n, A.ptr, lda, p, threshold, out.ptr, ldo, stream))
+
+
+def dsyev(jobz, n, A: DeviceArray, lda, w: DeviceArray, stream=None) -> int:
+    """Eigenvalues (ascending) into w, eigenvectors into A for jobz 'V'.
+    Returns LAPACK's info."""
+    info = C.c_int(0)
+    check(lib().rt_dsyev_dev(jobz.encode(), n, A.ptr, lda, w.ptr, C.byref(info), stream))
+    return info.value
+
+
+def dsteqr_host(compz, d, e, z=None):
+    """Host DSTEQR (the tridiagonal stage of dsyev) on numpy arrays, in place:
+    d (n,), e (n-1,), z (n, n) Fortran order for compz 'I'/'V'.  Returns info."""
+    n = d.shape[0]
+    info = C.c_int(0)
+    zp = z.ctypes.data if z is not None else None
+    ldz = z.shape[0] if z is not None else max(n, 1)
+    check(lib().rt_dsteqr_host(compz.encode(), n, d.ctypes.data, e.ctypes.data if n > 1 else None,
+                               zp, ldz, C.byref(info)))
+    return info.value

@@ -102,8 +102,7 @@ def test_matrix_power(D, p, n):
 
 @pytest.mark.parametrize("p", [-0.5, 0.5])
 def test_matrix_power_ill_conditioned(D, p):
-    """cond = 1e6: the square roots go through the Cholesky factor's polar
-    factor, which sees only sqrt(cond)."""
+    """cond = 1e6, nothing dropped."""
     n = 400
     S = M.spd(n, seed=99, lo=1e-6, hi=1.0)
     dA, dO = D.DeviceArray.from_host(S), D.DeviceArray(n * n)
@@ -113,8 +112,12 @@ def test_matrix_power_ill_conditioned(D, p):
     assert np.array_equal(out, out.T)
 
 
-def test_matrix_power_threshold_guard(D):
-    from rest_tensors_amd._lib import RtError
-    S = np.diag([4.0, 1e-12, 2.0])             # the reference would drop 1e-12
-    with pytest.raises(RtError):
-        D.matrix_power(3, D.DeviceArray.from_host(S), 3, -0.5, 1e-10, D.DeviceArray(9), 3)
+def test_matrix_power_threshold_drop(D):
+    """The reference drops eigenvalues below the threshold: 1e-12 is dropped
+    and its eigenvector contributes nothing."""
+    S = np.diag([4.0, 1e-12, 2.0])
+    dO = D.DeviceArray(9)
+    D.matrix_power(3, D.DeviceArray.from_host(S), 3, -0.5, 1e-10, dO, 3)
+    out = dO.to_host((3, 3))
+    assert np.abs(out - np.diag([0.5, 0.0, 2 ** -0.5])).max() < 1e-14
+    assert np.abs(out - M.lapack_power(S, -0.5, 1e-10)).max() < 1e-14
