@@ -1,8 +1,8 @@
 // Tests of the C++ host mirror (include/rest_tensors.hpp), written after the
 // reference's own examples: the doc examples of MatrixFull / _dgemm
 // (src/matrix/mod.rs:437-452, src/matrix/matrix_blas_lapack.rs:76-119), the
-// ao2mo benchmark shape (benches/bench_tensors.rs:5-16) and a Davidson solve
-// judged by its residual.  `--cpu-only` runs the checks that touch no GPU.
+// ao2mo benchmark shape (benches/bench_tensors.rs:5-16), a Davidson solve
+// judged by its residual, and the _dsyev doc example (matrix_blas_lapack.rs:286-318).  `--cpu-only` runs the checks that touch no GPU.
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -161,6 +161,36 @@ static void gpu_checks() {
   // a positive rank-one update interlaces: e_k lies in [d_k, d_{k+1}] = [k+1, k+2]
   CHECK("davidson_interlacing", e[0] >= 1.0 && e[0] <= 2.0 && e[1] >= 2.0 && e[1] <= 3.0 &&
                                     e[2] >= 3.0 && e[2] <= 4.0);
+
+  // _dsyev doc example (matrix_blas_lapack.rs:286-318), eigenvectors sign-sensitive
+  {
+    auto up = MatrixUpper<double>::from_vec(6, seq(1, 7));
+    auto a = *up.to_matrixfull();
+    auto [vec, w, nd] = rest::_dsyev(a, 'V');
+    const double wref[3] = {-1.5066326307865059, -0.05739624271478554, 11.564028873501286};
+    const double vref[9] = {-0.6827362941552275, -0.38559063640162244, 0.6206375864887483,
+                            0.6202872696512856,  -0.7547821848190948,  0.21341873532627673,
+                            -0.3861539275363389, -0.5306823104260265,  -0.7544941548144386};
+    double dv = 0.0, dw = 0.0;
+    for (int k = 0; k < 9; ++k) dv += (vec->data[k] - vref[k]) * (vec->data[k] - vref[k]);
+    for (int k = 0; k < 3; ++k) dw += (w[k] - wref[k]) * (w[k] - wref[k]);
+    CHECK("dsyev_doc_example", nd == 3 && dv < 1e-6 && dw < 1e-6);
+    auto [none, w2, nd2] = rest::_dsyev(a, 'N');
+    CHECK("dsyev_doc_example_values_only", !none.has_value() && std::fabs(w2[2] - wref[2]) < 1e-12);
+    // _power(-0.5): X A X = I (A has eigenvalues of both signs: the negative
+    // ones fall below the threshold and are dropped, so only the projector
+    // onto the positive eigenvector remains)
+    auto x = rest::_power(a, -0.5, 1e-10);
+    double xax = 0.0;  // u^T (X A X) u for the kept eigenvector u = column 2
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) {
+        double s2 = 0.0;
+        for (int k = 0; k < 3; ++k)
+          for (int l = 0; l < 3; ++l) s2 += (*x)(i, k) * a(k, l) * (*x)(l, j);
+        xax += vec->data[6 + i] * s2 * vec->data[6 + j];
+      }
+    CHECK("power_drops_negative_eigenvalues", std::fabs(xax - 1.0) < 1e-12);
+  }
 }
 
 int main(int argc, char** argv) {
