@@ -192,56 +192,97 @@ struct EvSweep {
   int64_t off;
 };
 
-// One thread per row of Z.  The column a rotation leaves for the next one is
-// carried in a register; the other column is loaded 8 rotations ahead.
+// One workgroup (one wave) per 64 rows of Z, one thread per row.  The column
+// a rotation leaves for the next one is carried in a register; the column it
+// takes in is loaded ROT_P - 1 chunks of ROT_K rotations ahead (a register
+// ring), so several loads are in flight while the chain advances.  A sweep's
+// (c, s) pairs are staged in LDS, ROT_SEG at a time, by the whole wave.
+constexpr int ROT_K = 8, ROT_P = 4, ROT_SEG = 2048;
+
+// rotations t in [t0, t1) of one sweep; t counts in the sweep's own order
+// (backward sweeps: t = 0 is j = len - 1).  qs holds the pairs of this
+// segment, qs[t - t0] in that order.
+template <bool FWD>
+__device__ __forceinline__ void ev_rot_segment(double* __restrict__ z, int64_t ldz, int64_t l,
+                                               int len, int t0, int t1,
+                                               const double2* __restrict__ qs,
+                                               double& carry) {
+  // load index of rotation t: the column it takes in
+  const int64_t base = FWD ? l + 1 : l + len - 1;
+  auto col = [&](int t) { return FWD ? base + t : base - t; };
+  const int cnt = t1 - t0;
+  auto rot = [&](int t, double a) {  // rotation t of the segment, column value a
+    const double2 q = qs[t];
+    const int64_t c = col(t0 + t);
+    if (FWD) {
+      z[(c - 1) * ldz] = q.y * a + q.x * carry;
+      carry = q.x * a - q.y * carry;
+    } else {
+      z[(c + 1) * ldz] = q.x * carry - q.y * a;
+      carry = q.y * carry + q.x * a;
+    }
+  };
+  double buf[ROT_P][ROT_K];
+  auto load = [&](int ch, double (&b)[ROT_K]) {
+#pragma unroll
+    for (int u = 0; u < ROT_K; ++u) b[u] = z[col(t0 + ch * ROT_K + u) * ldz];
+  };
+  auto comp = [&](int ch, const double (&b)[ROT_K]) {
+#pragma unroll
+    for (int u = 0; u < ROT_K; ++u) rot(ch * ROT_K + u, b[u]);
+  };
+  // full chunks: a branch-free steady state keeps chunks c .. c+P-2 in flight
+  // (buf[p] holds chunk c + p at the top of each pass; <= 48 memory operations
+  // outstanding, within the 6-bit vmcnt)
+  const int nfull = cnt / ROT_K;
+  int c = 0;
+  if (nfull >= ROT_P - 1) {
+#pragma unroll
+    for (int p = 0; p < ROT_P - 1; ++p) load(p, buf[p]);
+    for (; c + 2 * ROT_P - 2 < nfull; c += ROT_P) {
+#pragma unroll
+      for (int p = 0; p < ROT_P; ++p) {
+        load(c + p + ROT_P - 1, buf[(p + ROT_P - 1) % ROT_P]);
+        comp(c + p, buf[p]);
+      }
+    }
+    // the loaded chunks c .. c+P-2 (those that exist), then the rest
+#pragma unroll
+    for (int p = 0; p < ROT_P - 1; ++p)
+      if (c + p < nfull) comp(c + p, buf[p]);
+    c += ROT_P - 1;
+  }
+  for (int t = c * ROT_K; t < cnt; ++t) rot(t, z[col(t0 + t) * ldz]);
+}
+
 __global__ __launch_bounds__(64) void ev_rot_kernel(double* __restrict__ Z, int64_t ldz,
                                                     int64_t nrow,
                                                     const EvSweep* __restrict__ sw, int nsw,
                                                     const double2* __restrict__ cs) {
-  const int64_t r = blockIdx.x * (int64_t)64 + threadIdx.x;
-  if (r >= nrow) return;
-  double* z = Z + r;
+  __shared__ double2 qs[ROT_SEG];
+  const int lane = threadIdx.x;
+  const int64_t r = blockIdx.x * (int64_t)64 + lane;
+  const bool active = r < nrow;
+  double* z = Z + (active ? r : 0);
   for (int k = 0; k < nsw; ++k) {
     const EvSweep S = sw[k];
-    const double2* q = cs + S.off;
     const int64_t l = S.l;
     const int len = S.len;
-    if (S.fwd) {
-      double carry = z[l * ldz];
-      for (int j0 = 0; j0 < len; j0 += 8) {
-        const int cnt = (len - j0 < 8) ? len - j0 : 8;
-        double a[8];
-#pragma unroll
-        for (int t = 0; t < 8; ++t)
-          if (t < cnt) a[t] = z[(l + j0 + t + 1) * ldz];
-#pragma unroll
-        for (int t = 0; t < 8; ++t) {
-          if (t < cnt) {
-            const double2 cq = q[j0 + t];
-            z[(l + j0 + t) * ldz] = cq.y * a[t] + cq.x * carry;
-            carry = cq.x * a[t] - cq.y * carry;
-          }
-        }
-      }
-      z[(l + len) * ldz] = carry;
-    } else {
-      double carry = z[(l + len) * ldz];
-      for (int j0 = len - 1; j0 >= 0; j0 -= 8) {
-        const int cnt = (j0 + 1 < 8) ? j0 + 1 : 8;
-        double a[8];
-#pragma unroll
-        for (int t = 0; t < 8; ++t)
-          if (t < cnt) a[t] = z[(l + j0 - t) * ldz];
-#pragma unroll
-        for (int t = 0; t < 8; ++t) {
-          if (t < cnt) {
-            const double2 cq = q[j0 - t];
-            z[(l + j0 - t + 1) * ldz] = cq.x * carry - cq.y * a[t];
-            carry = cq.y * carry + cq.x * a[t];
-          }
-        }
-      }
-      z[l * ldz] = carry;
+    double carry = 0.0;
+    if (active) carry = S.fwd ? z[l * ldz] : z[(l + len) * ldz];
+    for (int t0 = 0; t0 < len; t0 += ROT_SEG) {
+      const int t1 = (t0 + ROT_SEG < len) ? t0 + ROT_SEG : len;
+      __syncthreads();  // the previous segment's pairs are consumed
+      for (int t = t0 + lane; t < t1; t += 64)
+        qs[t - t0] = S.fwd ? cs[S.off + t] : cs[S.off + (len - 1 - t)];
+      __syncthreads();
+      if (!active) continue;
+      if (S.fwd) ev_rot_segment<true>(z, ldz, l, len, t0, t1, qs, carry);
+      else ev_rot_segment<false>(z, ldz, l, len, t0, t1, qs, carry);
+    }
+    if (active) {
+      if (S.fwd) z[(l + len) * ldz] = carry;
+      else z[l * ldz] = carry;
     }
   }
 }
