@@ -229,6 +229,13 @@ int rt_upper_pack_dev(const double* full, int64_t n, double* packed,
  * pack per aux slice (rifull_to_matfull_symm, ri.rs:297-306, 330-341). */
 int rt_rifull_to_matfull_symm_dev(const double* ri, int64_t nao, int64_t naux,
                                   double* out, void* stream);
+/* Index permutations of a column-major [n0, n1, n2] RIFull (ri.rs:225-290):
+ * order "jik" -> [n1, n0, n2] (transpose_jik), "jki" -> [n1, n2, n0]
+ * (transpose_jki), "kji" -> [n2, n1, n0] (transpose_kji), "ikj" ->
+ * [n0, n2, n1] (transpose_ikj).  Bit-exact copies; src and out must not
+ * overlap. */
+int rt_rifull_transpose_dev(const double* src, int64_t n0, int64_t n1, int64_t n2,
+                            const char* order, double* out, void* stream);
 /* integer dimension of a packed length (exact isqrt; -1 if not triangular) */
 int64_t rt_upper_dim(int64_t packed_len);
 

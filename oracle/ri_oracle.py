@@ -34,6 +34,11 @@ Pinning summary:
   ozaki_i8 (slices, K) no reference counterpart (restates the INT8 form);
                       pinned: reconstruction bound, brute-force level sums,
                       1e-8 against rik.
+  rifull_transpose    pinned: a literal restatement of the reference's
+                      chunk-transpose and index-filter loops (ri.rs:225-290) on
+                      its own test shape [3,2,2] and others, and the
+                      permutation group (jik twice, kji twice, jki three times
+                      are the identity).
   special_dgemm_01    parity unpinned against the reference (its Fortran
                       aliases dgemm's A and C, restmatr.f90:140-150); pinned
                       against the product formula it documents instead.
@@ -301,6 +306,49 @@ def upper_unpack(packed: np.ndarray) -> np.ndarray | None:
         for i in range(j + 1, n):
             full[i, j] = full[j, i]
     return full
+
+
+def rifull_transpose(ri: np.ndarray, order: str) -> np.ndarray:
+    """RIFull::transpose_{jik,jki,kji,ikj} (ri.rs:225-290) on a column-major
+    (n0, n1, n2) array: the result's axes are the named permutation."""
+    axes = {"jik": (1, 0, 2), "jki": (1, 2, 0), "kji": (2, 1, 0), "ikj": (0, 2, 1)}[order]
+    return np.asfortranarray(np.transpose(ri, axes))
+
+
+def rifull_transpose_literal(data: list, size: tuple, order: str) -> list:
+    """The reference's element order, restated step by step on a flat
+    column-major list (small sizes only): transpose_jik transposes each
+    n0 x n1 chunk (ri.rs:227-238); transpose_jki keeps, for each ii, the
+    elements whose flat index idx has |idx - ii| % n0 == 0 (ri.rs:241-256);
+    transpose_kji filters transpose_jik's output by |idx - ij| % (n0 n1) == 0
+    (ri.rs:259-275); transpose_ikj gathers column jj of every chunk
+    (ri.rs:278-293)."""
+    i, j, k = size
+    if order == "jik":
+        out = []
+        for c in range(k):
+            chunk = data[c * i * j:(c + 1) * i * j]
+            out += [chunk[a + i * b] for a in range(i) for b in range(j)]
+        return out
+    if order == "jki":
+        out = []
+        for ii in range(i):
+            out += [v for idx, v in enumerate(data) if abs(idx - ii) % i == 0]
+        return out
+    if order == "kji":
+        t = rifull_transpose_literal(data, size, "jik")
+        out = []
+        for ij in range(i * j):
+            out += [v for idx, v in enumerate(t) if abs(idx - ij) % (i * j) == 0]
+        return out
+    if order == "ikj":
+        out = []
+        for jj in range(j):
+            for c in range(k):
+                chunk = data[c * i * j:(c + 1) * i * j]
+                out += chunk[jj * i:(jj + 1) * i]
+        return out
+    raise ValueError(order)
 
 
 def rifull_to_matfull_symm(ri: np.ndarray) -> np.ndarray:

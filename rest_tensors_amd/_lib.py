@@ -71,6 +71,7 @@ RT_SIGNATURES = {
     "rt_upper_pack_dev": (i32, [vp, i64, vp, vp]),
     "rt_rifull_to_matfull_symm_dev": (i32, [vp, i64, i64, vp, vp]),
     "rt_upper_dim": (i64, [i64]),
+    "rt_rifull_transpose_dev": (i32, [vp, i64, i64, i64, C.c_char_p, vp, vp]),
     "rt_copy_mm_dev": (i32, [i64, i64, vp, i64, i64, i64, i64, vp, i64, i64, i64, i64, vp]),
     "rt_copy_mr_dev": (i32, [i64, i64, vp, i64, i64, i64, i64, vp, i64, i64, i64, i64, i64,
                              i64, i32, vp]),

@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <string>
 
 #include "common.hpp"
 #include "layout.hpp"
@@ -93,6 +94,47 @@ int copy3d(int64_t X, int64_t Y, int64_t Z, const double* src, int64_t sx,
       (unsigned)std::max<int64_t>(1, std::min<int64_t>((total + 255) / 256, 65536));
   hipLaunchKernelGGL(copy3d_kernel, dim3(grid), dim3(256), 0, s, X, Y, Z, src,
                      sx, sy, sz, dst, dx, dy, dz);
+  RTA_HIP(hipGetLastError());
+  return RT_OK;
+}
+
+// Batched 2D transpose through LDS: out[b*ob + p*op + q] = in[b*ib + p + q*iq]
+// for p < P, q < Q, b < NB (the input runs along p, the output along q).  32 x 32
+// tiles, a grid-stride loop over (tile, batch); both sides move 256-byte rows.
+__global__ __launch_bounds__(256) void transpose_tile_kernel(
+    const double* __restrict__ in, int64_t P, int64_t Q, int64_t NB, int64_t iq, int64_t ib,
+    double* __restrict__ out, int64_t op, int64_t ob) {
+  __shared__ double T[32][33];
+  const int64_t tp = (P + 31) / 32, tq = (Q + 31) / 32;
+  const int64_t ntile = tp * tq * NB;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  for (int64_t t = blockIdx.x; t < ntile; t += gridDim.x) {
+    const int64_t b = t / (tp * tq), rem = t % (tp * tq);
+    const int64_t p0 = (rem % tp) * 32, q0 = (rem / tp) * 32;
+    const double* src = in + b * ib;
+    double* dst = out + b * ob;
+    __syncthreads();  // the previous tile's reads of T are done
+#pragma unroll
+    for (int r = ty; r < 32; r += 8) {
+      const int64_t p = p0 + tx, q = q0 + r;
+      if (p < P && q < Q) T[r][tx] = src[p + q * iq];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = ty; r < 32; r += 8) {
+      const int64_t q = q0 + tx, p = p0 + r;
+      if (p < P && q < Q) dst[p * op + q] = T[tx][r];
+    }
+  }
+}
+
+static int transpose_tiles(const double* in, int64_t P, int64_t Q, int64_t NB, int64_t iq,
+                           int64_t ib, double* out, int64_t op, int64_t ob, hipStream_t s) {
+  const int64_t ntile = ((P + 31) / 32) * ((Q + 31) / 32) * NB;
+  if (ntile <= 0) return RT_OK;
+  const unsigned grid = (unsigned)std::min<int64_t>(ntile, 262144);
+  hipLaunchKernelGGL(transpose_tile_kernel, dim3(grid), dim3(256), 0, s, in, P, Q, NB, iq, ib,
+                     out, op, ob);
   RTA_HIP(hipGetLastError());
   return RT_OK;
 }
@@ -284,3 +326,31 @@ extern "C" int rt_copy_rr_dev(int64_t x_len, int64_t y_len, int64_t z_len,
                 1, f_x, f_x * f_y, t + t_xs + t_ys * t_x + t_zs * t_x * t_y, 1,
                 t_x, t_x * t_y, (hipStream_t)stream);
 }
+
+// RIFull::transpose_jik / transpose_jki / transpose_kji / transpose_ikj
+// (ri.rs:225-290): every one is an index permutation of the column-major
+// [n0, n1, n2] tensor (the reference builds them from chunked matrix
+// transposes and index filters; the element order they produce is this one).
+extern "C" int rt_rifull_transpose_dev(const double* src, int64_t n0, int64_t n1, int64_t n2,
+                                       const char* order, double* out, void* stream) {
+  RTA_CHECK(order, "order is NULL");
+  RTA_CHECK(n0 >= 0 && n1 >= 0 && n2 >= 0, "negative dimension");
+  const std::string o(order);
+  RTA_CHECK(o == "jik" || o == "jki" || o == "kji" || o == "ikj",
+            "order must be \"jik\", \"jki\", \"kji\" or \"ikj\", got \"%s\"", order);
+  if (n0 * n1 * n2 == 0) return RT_OK;
+  RTA_CHECK(src && out, "NULL argument");
+  RTA_CHECK(src + n0 * n1 * n2 <= out || out + n0 * n1 * n2 <= src,
+            "src and out overlap (the transpose is not in place)");
+  hipStream_t s = (hipStream_t)stream;
+  KernelTimer kt("rifull_transpose", s);
+  if (o == "jik")  // [n1, n0, n2]: per k, an n0 x n1 transpose
+    return transpose_tiles(src, n0, n1, n2, n0, n0 * n1, out, n1, n0 * n1, s);
+  if (o == "jki")  // [n1, n2, n0]: one n0 x (n1 n2) transpose
+    return transpose_tiles(src, n0, n1 * n2, 1, n0, 0, out, n1 * n2, 0, s);
+  if (o == "kji")  // [n2, n1, n0]: per j, an n0 x n2 transpose
+    return transpose_tiles(src, n0, n2, n1, n0 * n1, n0, out, n2 * n1, n2, s);
+  // ikj: [n0, n2, n1], columns of length n0 move as units
+  return copy3d(n0, n1, n2, src, 1, n0, n0 * n1, out, 1, n0 * n2, n0, s);
+}
+

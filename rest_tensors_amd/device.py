@@ -246,6 +246,11 @@ def matrix_power(n, A: DeviceArray, lda, p, threshold, out: DeviceArray, ldo, st
     check(lib().rt_matrix_power_dev(n, A.ptr, lda, p, threshold, out.ptr, ldo, stream))
 
 
+def rifull_transpose(src: DeviceArray, n0, n1, n2, order: str, out: DeviceArray, stream=None):
+    """RIFull::transpose_<order> of the column-major [n0, n1, n2] tensor."""
+    check(lib().rt_rifull_transpose_dev(src.ptr, n0, n1, n2, order.encode(), out.ptr, stream))
+
+
 def dsyev(jobz, n, A: DeviceArray, lda, w: DeviceArray, stream=None) -> int:
     """Eigenvalues (ascending) into w, eigenvectors into A for jobz 'V'.
     Returns LAPACK's info."""

@@ -162,6 +162,22 @@ static void gpu_checks() {
   CHECK("davidson_interlacing", e[0] >= 1.0 && e[0] <= 2.0 && e[1] >= 2.0 && e[1] <= 3.0 &&
                                     e[2] >= 3.0 && e[2] <= 4.0);
 
+  // transposes on the reference's test tensor [3,2,2] of 0..11 (ri.rs:437-449)
+  {
+    auto ri = *RIFull<double>::from_vec({3, 2, 2}, seq(0, 12));
+    auto a = ri.transpose_jik(), b = ri.transpose_jki(), c = ri.transpose_kji(),
+         d = ri.transpose_ikj();
+    bool ok = a.size == std::array<size_t, 3>{2, 3, 2} && b.size == std::array<size_t, 3>{2, 2, 3} &&
+              c.size == std::array<size_t, 3>{2, 2, 3} && d.size == std::array<size_t, 3>{3, 2, 2};
+    for (size_t x = 0; x < 3; ++x)
+      for (size_t y = 0; y < 2; ++y)
+        for (size_t z = 0; z < 2; ++z) {
+          const double v = ri(x, y, z);
+          ok = ok && a(y, x, z) == v && b(y, z, x) == v && c(z, y, x) == v && d(x, z, y) == v;
+        }
+    CHECK("rifull_transposes", ok);
+  }
+
   // _dsyev doc example (matrix_blas_lapack.rs:286-318), eigenvectors sign-sensitive
   {
     auto up = MatrixUpper<double>::from_vec(6, seq(1, 7));

@@ -333,3 +333,39 @@ def test_rifull_to_matfull_symm_bit_exact(D):
     out = D.DeviceArray(nao * (nao + 1) // 2 * naux)
     D.rifull_to_matfull_symm(d, nao, naux, out)
     assert np.array_equal(out.to_host((nao * (nao + 1) // 2, naux)), O.rifull_to_matfull_symm(ri))
+
+
+@pytest.mark.parametrize("order", ["jik", "jki", "kji", "ikj"])
+@pytest.mark.parametrize("size", [(3, 2, 2), (1, 1, 1), (33, 65, 7), (130, 70, 33), (1, 300, 5),
+                                  (257, 1, 3), (64, 64, 64)])
+def test_rifull_transpose_bit_exact(D, order, size):
+    """RIFull::transpose_* (ri.rs:225-290) against the oracle, bit for bit."""
+    print(f"[transpose] {order} {size}", flush=True)
+    rng = np.random.default_rng(0x5EED7500)
+    ri = np.asfortranarray(rng.standard_normal(size))
+    ref = O.rifull_transpose(ri, order)
+    dsrc = D.DeviceArray.from_host(ri)
+    dout = D.DeviceArray(ri.size)
+    D.rifull_transpose(dsrc, *size, order, dout)
+    assert np.array_equal(dout.to_host(ref.shape), ref)
+
+
+def test_rifull_transpose_ri3ao_round_trip(D):
+    """A C2-sized ri3ao slab: jki three times is the identity."""
+    nb, naux = 512, 64
+    d0 = D.DeviceArray(nb * nb * naux)
+    D.gen_ri3ao(d0, nb, naux, seed=0x5EED7501)
+    d1, d2 = D.DeviceArray(nb * nb * naux), D.DeviceArray(nb * nb * naux)
+    D.rifull_transpose(d0, nb, nb, naux, "jki", d1)      # [nb, naux, nb]
+    D.rifull_transpose(d1, nb, naux, nb, "jki", d2)      # [naux, nb, nb]
+    D.rifull_transpose(d2, naux, nb, nb, "jki", d1)      # [nb, nb, naux]
+    assert np.array_equal(d1.to_host((nb * nb * naux,)), d0.to_host((nb * nb * naux,)))
+
+
+def test_rifull_transpose_rejects(D):
+    from rest_tensors_amd._lib import RtError
+    a = D.DeviceArray(8)
+    with pytest.raises(RtError):
+        D.rifull_transpose(a, 2, 2, 2, "kij", D.DeviceArray(8))
+    with pytest.raises(RtError):
+        D.rifull_transpose(a, 2, 2, 2, "jik", a)          # in place is refused

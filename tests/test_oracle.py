@@ -399,3 +399,28 @@ def test_metric_closed_forms():
     Mf = M.cholesky_decompose_inverse(D, "L")
     assert np.allclose(Mf, np.diag([0.5, 1 / 3, 2.0]), rtol=1e-15)
     assert M.cholesky_decompose_inverse(np.diag([1.0, -1.0]), "L") is None
+
+
+# ---- RIFull transposes (ri.rs:225-290) ----------------------------------------
+
+@pytest.mark.parametrize("size", [(3, 2, 2), (9, 3, 2), (1, 4, 3), (5, 1, 7), (4, 6, 1), (2, 3, 5)])
+@pytest.mark.parametrize("order", ["jik", "jki", "kji", "ikj"])
+def test_rifull_transpose_literal(size, order):
+    """numpy permutation == the reference's own loops, on its test shape [3,2,2]
+    (ri.rs:437-449) and others."""
+    n = size[0] * size[1] * size[2]
+    data = list(range(n))
+    lit = O.rifull_transpose_literal(data, size, order)
+    ri = np.asfortranarray(np.arange(n, dtype=float).reshape(size, order="F"))
+    out = O.rifull_transpose(ri, order)
+    assert out.ravel(order="F").tolist() == [float(v) for v in lit]
+
+
+def test_rifull_transpose_group():
+    ri = np.asfortranarray(np.random.default_rng(3).standard_normal((4, 5, 6)))
+    T = O.rifull_transpose
+    assert np.array_equal(T(T(ri, "jik"), "jik"), ri)
+    assert np.array_equal(T(T(ri, "kji"), "kji"), ri)
+    assert np.array_equal(T(T(ri, "ikj"), "ikj"), ri)
+    assert np.array_equal(T(T(T(ri, "jki"), "jki"), "jki"), ri)
+    assert T(ri, "jki").shape == (5, 6, 4) and T(ri, "kji").shape == (6, 5, 4)
