@@ -210,6 +210,25 @@ int rt_matrix_power_dev(int64_t n, const double* A, int64_t lda, double p,
  * Synchronizes. */
 int rt_dsyev_dev(char jobz, int64_t n, double* A, int64_t lda, double* w, int* info,
                  void* stream);
+/* Generalized eigenproblem A x = lambda B x (itype 1) on packed-upper A, B
+ * (j(j+1)/2 + i): the lowest num_orb eigenpairs, Z [n, num_orb] with
+ * Z^T B Z = I, w[n] ascending with w[num_orb..n) = 0 (the reference's w).
+ * info: 0, n + k when B's leading minor of order k is not positive definite
+ * (the reference panics), else the eigensolver's.  Replaces _dspgvx /
+ * lapack_dspgvx (src/matrix/matrix_blas_lapack.rs:2123-2186, 1096-1147).
+ * Synchronizes. */
+int rt_dspgvx_dev(int64_t n, const double* Ap, const double* Bp, int64_t num_orb, double* Z,
+                  double* w, int* info, void* stream);
+/* All eigenpairs of a packed-upper symmetric matrix: Z [n, n], w[n]
+ * ascending, *n_found = n.  Replaces lapack_dspevx
+ * (src/matrix/matrix_blas_lapack.rs:1075-1095).  Synchronizes. */
+int rt_dspevx_dev(int64_t n, const double* Ap, double* Z, double* w, int* n_found,
+                  void* stream);
+/* Packed RHF Fock matrix F = H + J - K/2 (upper triangle) from full n x n
+ * H, J, K.  No counterpart in the reference crate (the SCF driver that
+ * assembles it lives outside); it feeds rt_dspgvx_dev. */
+int rt_fock_rhf_packed_dev(int64_t n, const double* H, const double* J, const double* K,
+                           double* F, void* stream);
 /* Host-only: LAPACK DSTEQR on host arrays (compz 'N', 'I' or 'V'; d[n],
  * e[n-1], z[n x n] with ld ldz), the tridiagonal stage of rt_dsyev_dev. */
 int rt_dsteqr_host(char compz, int64_t n, double* d, double* e, double* z, int64_t ldz,

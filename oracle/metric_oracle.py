@@ -6,6 +6,8 @@ lapack crate), the sequences of
   cholesky_decompose_inverse  src/matrix/matrix_blas_lapack.rs:931-1002
   _dsyev                      src/matrix/matrix_blas_lapack.rs:319-348 (DSYEV, uplo 'L')
   lapack_power                src/matrix/matrix_blas_lapack.rs:1004-1064
+  _dspgvx / lapack_dspgvx     src/matrix/matrix_blas_lapack.rs:2123-2186, 1096-1147
+  lapack_dspevx               src/matrix/matrix_blas_lapack.rs:1075-1095
 and DSTEQR with vectors from the identity (through DSTEV), the tridiagonal
 stage of DSYEV.  The product never imports this module.
 
@@ -17,7 +19,9 @@ eigenvectors sin(j k pi/(n+1)) up to sign); lapack_power(-0.5) by X S X = I
 and cholesky_decompose_inverse by M M^T = S^-1 on the 8x8 matrix of the
 reference's own test (src/lib.rs:170-190), and both against the closed forms
 on diagonal and 2x2 matrices; the threshold drop of lapack_power against a
-matrix with known eigenpairs.
+matrix with known eigenpairs; dspgvx by its closed forms (B = I gives dsyev's
+eigenvalues; diagonal A, B give a_ii/b_ii with vectors e_i/sqrt(b_ii)) and by
+A Z = B Z W, Z^T B Z = I.
 """
 from __future__ import annotations
 
@@ -63,6 +67,39 @@ def dstev(d: np.ndarray, e: np.ndarray):
         return d.copy(), np.ones((1, 1)), 0
     w, z, info = lapack.dstev(d, np.array(e, float), compute_v=1)
     return w, z, int(info)
+
+
+def _unpack_upper(packed: np.ndarray, n: int) -> np.ndarray:
+    A = np.zeros((n, n))
+    iu = np.triu_indices(n)
+    # packed order: column j, rows 0..=j  (j(j+1)/2 + i)
+    order = np.argsort(iu[1] * (iu[1] + 1) // 2 + iu[0])
+    A[iu[0][order], iu[1][order]] = packed
+    return np.triu(A) + np.triu(A, 1).T
+
+
+def dspgvx(Ap: np.ndarray, Bp: np.ndarray, n: int, num_orb: int):
+    """_dspgvx (src/matrix/matrix_blas_lapack.rs:2123-2186): DSPGVX itype 1,
+    jobz 'V', range 'I' (1..num_orb), uplo 'U' on packed A, B; restated with
+    LAPACK's DSYGVX on the unpacked matrices (the same algorithm on full
+    storage).  Returns (Z [n, num_orb], w [n] with zeros after num_orb)."""
+    import scipy.linalg as sl
+    A, B = _unpack_upper(Ap, n), _unpack_upper(Bp, n)
+    w = np.zeros(n)
+    if num_orb == 0:
+        return np.zeros((n, 0)), w
+    ww, Z = sl.eigh(A, B, lower=False, driver="gvx", subset_by_index=[0, num_orb - 1])
+    w[:num_orb] = ww
+    return Z, w
+
+
+def dspevx(Ap: np.ndarray, n: int):
+    """lapack_dspevx (src/matrix/matrix_blas_lapack.rs:1075-1095): all
+    eigenpairs of a packed-upper matrix (DSYEVX on the unpacked one)."""
+    import scipy.linalg as sl
+    A = _unpack_upper(Ap, n)
+    w, Z = sl.eigh(A, lower=False, driver="evx")
+    return Z, w
 
 
 def lapack_power(A: np.ndarray, p: float, threshold: float):

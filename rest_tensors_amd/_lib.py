@@ -67,6 +67,9 @@ RT_SIGNATURES = {
     "rt_matrix_power_dev": (i32, [i64, vp, i64, f64, f64, vp, i64, vp]),
     "rt_dsyev_dev": (i32, [ch, i64, vp, i64, vp, pi32, vp]),
     "rt_dsteqr_host": (i32, [ch, i64, vp, vp, vp, i64, pi32]),
+    "rt_dspgvx_dev": (i32, [i64, vp, vp, i64, vp, vp, pi32, vp]),
+    "rt_dspevx_dev": (i32, [i64, vp, vp, vp, pi32, vp]),
+    "rt_fock_rhf_packed_dev": (i32, [i64, vp, vp, vp, vp, vp]),
     "rt_upper_unpack_dev": (i32, [vp, i64, vp, vp]),
     "rt_upper_pack_dev": (i32, [vp, i64, vp, vp]),
     "rt_rifull_to_matfull_symm_dev": (i32, [vp, i64, i64, vp, vp]),

@@ -30,6 +30,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "metric.hpp"
 #include "timer.hpp"
 
 namespace rta {
@@ -327,6 +328,18 @@ __global__ void ev_scale_cols_kernel(double* __restrict__ X, int64_t n,
                                      const double* __restrict__ f) {
   const int64_t e = blockIdx.x * (int64_t)256 + threadIdx.x;
   if (e < n * n) X[e] *= f[e / n];
+}
+
+// F = H + J - K/2 over the upper triangle, packed (j(j+1)/2 + i): the RHF
+// Fock matrix from the core Hamiltonian and the J and K builds.
+__global__ void ev_fock_packed_kernel(const double* __restrict__ H, const double* __restrict__ J,
+                                      const double* __restrict__ K, int64_t n,
+                                      double* __restrict__ F) {
+  const int64_t j = blockIdx.y;
+  const int64_t i = blockIdx.x * (int64_t)256 + threadIdx.x;
+  if (j >= n || i > j) return;
+  const int64_t e = i + j * n;
+  F[j * (j + 1) / 2 + i] = (H[e] + J[e]) - 0.5 * K[e];
 }
 
 // ---- host: DSTEQR restatement ----------------------------------------------
@@ -982,3 +995,94 @@ extern "C" int rt_matrix_power_dev(int64_t n, const double* A, int64_t lda, doub
   RTA_HIP(hipStreamSynchronize(s));
   return RT_OK;
 }
+
+// A x = lambda B x for packed-upper A, B (itype 1): the lowest num_orb
+// eigenpairs with Z^T B Z = I (_dspgvx / lapack_dspgvx,
+// src/matrix/matrix_blas_lapack.rs:2123-2186, 1096-1147, which call LAPACK
+// DSPGVX with range 'I', il = 1, iu = num_orb).  B = L L^T, C = L^-1 A L^-T,
+// C Y = Y Lambda, Z = L^-T Y.  w[0..num_orb) receives the eigenvalues, the rest
+// of w[n] is zero.  info: LAPACK's (n + k when B's leading minor of order k is
+// not positive definite).  Synchronizes.
+extern "C" int rt_dspgvx_dev(int64_t n, const double* Ap, const double* Bp, int64_t num_orb,
+                             double* Z, double* w, int* info, void* stream) {
+  RTA_CHECK(info, "info is NULL");
+  *info = 0;
+  RTA_CHECK(n >= 0 && num_orb >= 0 && num_orb <= n, "need 0 <= num_orb <= n (got %lld, %lld)",
+            (long long)num_orb, (long long)n);
+  if (n == 0) return RT_OK;
+  RTA_CHECK(Ap && Bp && Z && w, "NULL argument");
+  hipStream_t s = (hipStream_t)stream;
+  KernelTimer kt("dspgvx", s);
+  EvBufs b;
+  double *Af, *Lf, *X, *T1, *C, *Y;
+  int* dinfo;
+  RTA_TRY(b.take((size_t)n * n, &Af));
+  RTA_TRY(b.take((size_t)n * n, &Lf));
+  RTA_TRY(b.take((size_t)n * n, &X));
+  RTA_TRY(b.take((size_t)n * n, &T1));
+  RTA_TRY(b.take((size_t)n * n, &C));
+  RTA_TRY(b.take((size_t)n * n, &Y));
+  RTA_TRY(b.take((size_t)1, &dinfo));
+  const int64_t plen = n * (n + 1) / 2;
+  RTA_TRY(rt_upper_unpack_dev(Ap, plen, Af, s));
+  RTA_TRY(rt_upper_unpack_dev(Bp, plen, Lf, s));
+  int cinfo = 0;
+  RTA_TRY(chol_lower(Lf, n, &cinfo, dinfo, s));
+  if (cinfo != 0) {
+    *info = (int)n + cinfo;
+    return RT_OK;
+  }
+  RTA_TRY(tri_inverse_lower(Lf, n, X, s));                                          // L^-1
+  RTA_TRY(rt_dgemm_dev('N', 'N', n, n, n, 1.0, X, n, Af, n, 0.0, T1, n, s));        // L^-1 A
+  RTA_TRY(rt_dgemm_dev('N', 'T', n, n, n, 1.0, T1, n, X, n, 0.0, C, n, s));         // .. L^-T
+  std::vector<double> wh(n);
+  RTA_TRY(syev_device(n, C, n, 1.0, wh.data(), Y, n, info, s));
+  if (*info != 0) return RT_OK;
+  if (num_orb > 0)
+    RTA_TRY(rt_dgemm_dev('T', 'N', n, num_orb, n, 1.0, X, n, Y, n, 0.0, Z, n, s));  // L^-T Y
+  for (int64_t j = num_orb; j < n; ++j) wh[j] = 0.0;
+  RTA_HIP(hipMemcpyAsync(w, wh.data(), sizeof(double) * n, hipMemcpyHostToDevice, s));
+  RTA_HIP(hipStreamSynchronize(s));
+  return RT_OK;
+}
+
+// All eigenpairs of a packed-upper symmetric matrix (lapack_dspevx,
+// src/matrix/matrix_blas_lapack.rs:1075-1095: DSPEVX with range 'A'):
+// Z (n x n) and ascending w[n]; *n_found = n.  Synchronizes.
+extern "C" int rt_dspevx_dev(int64_t n, const double* Ap, double* Z, double* w, int* n_found,
+                             void* stream) {
+  RTA_CHECK(n_found, "n_found is NULL");
+  *n_found = 0;
+  RTA_CHECK(n >= 0, "negative dimension");
+  if (n == 0) return RT_OK;
+  RTA_CHECK(Ap && Z && w, "NULL argument");
+  hipStream_t s = (hipStream_t)stream;
+  EvBufs b;
+  double* Af;
+  RTA_TRY(b.take((size_t)n * n, &Af));
+  RTA_TRY(rt_upper_unpack_dev(Ap, n * (n + 1) / 2, Af, s));
+  std::vector<double> wh(n);
+  int info = 0;
+  RTA_TRY(syev_device(n, Af, n, 1.0, wh.data(), Z, n, &info, s));
+  if (info != 0) return fail(RT_ERR_NOTCONV, "dspevx: %d eigenvalues did not converge", info);
+  RTA_HIP(hipMemcpyAsync(w, wh.data(), sizeof(double) * n, hipMemcpyHostToDevice, s));
+  RTA_HIP(hipStreamSynchronize(s));
+  *n_found = (int)n;
+  return RT_OK;
+}
+
+// Packed RHF Fock matrix F = H + J - K/2 (upper triangle, j(j+1)/2 + i) from
+// full n x n H, J, K: the input the generalized eigensolve takes.
+extern "C" int rt_fock_rhf_packed_dev(int64_t n, const double* H, const double* J,
+                                      const double* K, double* F, void* stream) {
+  RTA_CHECK(n >= 0, "negative dimension");
+  if (n == 0) return RT_OK;
+  RTA_CHECK(H && J && K && F, "NULL argument");
+  RTA_CHECK(n <= 65535, "n = %lld exceeds the grid's y extent", (long long)n);
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(ev_fock_packed_kernel, dim3((unsigned)((n + 255) / 256), (unsigned)n),
+                     dim3(256), 0, s, H, J, K, n, F);
+  RTA_HIP(hipGetLastError());
+  return RT_OK;
+}
+

@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "metric.hpp"
 #include "timer.hpp"
 
 namespace rta {
@@ -141,7 +142,7 @@ __global__ void mt_eye_block_kernel(double* __restrict__ X, int64_t n, int64_t k
 // W (ld n) <- lower Cholesky factor of W in its lower triangle (the strict
 // upper triangle is left holding intermediate values); info (host) = first
 // non-positive pivot (1-based) or 0.
-static int chol_lower(double* W, int64_t n, int* info_h, int* info_d, hipStream_t s) {
+int chol_lower(double* W, int64_t n, int* info_h, int* info_d, hipStream_t s) {
   RTA_HIP(hipMemsetAsync(info_d, 0, sizeof(int), s));
   for (int64_t k0 = 0; k0 < n; k0 += CH_NB) {
     const int kb = (int)std::min<int64_t>(CH_NB, n - k0);
@@ -163,7 +164,7 @@ static int chol_lower(double* W, int64_t n, int* info_h, int* info_d, hipStream_
 
 // X (ld n) <- L^-1 for the lower factor in Lf (ld n): block rows in order,
 // X[b,:] = L_bb^-1 (I_b - L[b, :k0] X[:k0, :])
-static int tri_inverse_lower(const double* Lf, int64_t n, double* X, hipStream_t s) {
+int tri_inverse_lower(const double* Lf, int64_t n, double* X, hipStream_t s) {
   hipLaunchKernelGGL(mt_fill_kernel, dim3(mt_grid(n * n)), dim3(256), 0, s, X, n * n, 0.0);
   for (int64_t k0 = 0; k0 < n; k0 += CH_NB) {
     const int kb = (int)std::min<int64_t>(CH_NB, n - k0);

@@ -259,6 +259,26 @@ def dsyev(jobz, n, A: DeviceArray, lda, w: DeviceArray, stream=None) -> int:
     return info.value
 
 
+def dspgvx(n, Ap: DeviceArray, Bp: DeviceArray, num_orb, Z: DeviceArray, w: DeviceArray,
+           stream=None) -> int:
+    """A x = lambda B x on packed-upper A, B; returns LAPACK's info."""
+    info = C.c_int(0)
+    check(lib().rt_dspgvx_dev(n, Ap.ptr, Bp.ptr, num_orb, Z.ptr, w.ptr, C.byref(info), stream))
+    return info.value
+
+
+def dspevx(n, Ap: DeviceArray, Z: DeviceArray, w: DeviceArray, stream=None) -> int:
+    """All eigenpairs of a packed-upper matrix; returns the number found."""
+    nf = C.c_int(0)
+    check(lib().rt_dspevx_dev(n, Ap.ptr, Z.ptr, w.ptr, C.byref(nf), stream))
+    return nf.value
+
+
+def fock_rhf_packed(n, H: DeviceArray, J: DeviceArray, K: DeviceArray, F: DeviceArray,
+                    stream=None):
+    check(lib().rt_fock_rhf_packed_dev(n, H.ptr, J.ptr, K.ptr, F.ptr, stream))
+
+
 def dsteqr_host(compz, d, e, z=None):
     """Host DSTEQR (the tridiagonal stage of dsyev) on numpy arrays, in place:
     d (n,), e (n-1,), z (n, n) Fortran order for compz 'I'/'V'.  Returns info."""

@@ -178,6 +178,19 @@ static void gpu_checks() {
     CHECK("rifull_transposes", ok);
   }
 
+  // _dspgvx with B = diag(2, 2, 2): eigenvalues of A / 2, Z^T B Z = I
+  {
+    auto A = MatrixUpper<double>::from_vec(6, {4.0, 1.0, 6.0, 0.0, 1.0, 8.0});
+    auto Bm = MatrixUpper<double>::from_vec(6, {2.0, 0.0, 2.0, 0.0, 0.0, 2.0});
+    auto res = rest::_dspgvx(A, Bm, 2);
+    auto [vec_a, w_a, n_a] = rest::lapack_dspevx(A);
+    bool ok = res.has_value() && n_a == 3 && std::fabs(res->second[0] - 0.5 * w_a[0]) < 1e-13 &&
+              std::fabs(res->second[1] - 0.5 * w_a[1]) < 1e-13 && res->second[2] == 0.0;
+    double g = 0.0;  // (Z^T B Z)(0,0) with B = 2 I
+    for (int r = 0; r < 3; ++r) g += 2.0 * res->first(r, 0) * res->first(r, 0);
+    CHECK("dspgvx_scaled_overlap", ok && std::fabs(g - 1.0) < 1e-13);
+  }
+
   // _dsyev doc example (matrix_blas_lapack.rs:286-318), eigenvectors sign-sensitive
   {
     auto up = MatrixUpper<double>::from_vec(6, seq(1, 7));

@@ -114,3 +114,34 @@ def test_oracle_lapack_power_threshold_drop():
     keep = lam >= 1e-6
     ref = (Q[:, keep] * lam[keep] ** -0.5) @ Q[:, keep].T
     assert np.abs(X - ref).max() < 1e-9
+
+
+def _pack_upper(A):
+    n = A.shape[0]
+    return np.concatenate([A[:j + 1, j] for j in range(n)]) if n else np.zeros(0)
+
+
+def test_oracle_dspgvx_closed_forms():
+    rng = np.random.default_rng(11)
+    n = 9
+    X = rng.standard_normal((n, n))
+    A = 0.5 * (X + X.T)
+    # B = I: the standard problem
+    Z, w = M.dspgvx(_pack_upper(A), _pack_upper(np.eye(n)), n, 4)
+    _, wr, _ = M.dsyev(A, "N")
+    assert np.abs(w[:4] - wr[:4]).max() < 1e-13 and np.all(w[4:] == 0.0)
+    # diagonal A, B: lambda = a/b, vectors e_i / sqrt(b_i)
+    a = rng.standard_normal(n)
+    bb = rng.uniform(0.5, 2.0, n)
+    Z, w = M.dspgvx(_pack_upper(np.diag(a)), _pack_upper(np.diag(bb)), n, n)
+    order = np.argsort(a / bb)
+    assert np.abs(w - (a / bb)[order]).max() < 1e-14
+    ref = np.zeros((n, n))
+    ref[order, np.arange(n)] = 1.0 / np.sqrt(bb[order])
+    assert np.abs(np.abs(Z) - ref).max() < 1e-14
+    # general: residual and B-orthonormality
+    Y = rng.standard_normal((n, n))
+    B = Y @ Y.T + n * np.eye(n)
+    Z, w = M.dspgvx(_pack_upper(A), _pack_upper(B), n, 5)
+    assert np.abs(A @ Z - B @ Z * w[None, :5]).max() < 1e-12
+    assert np.abs(Z.T @ B @ Z - np.eye(5)).max() < 1e-12

@@ -144,3 +144,100 @@ def test_matrix_power_general_p_with_drop(D, p):
     keep = lam >= 1e-3                          # the exact answer from the known eigenpairs
     exact = (Q[:, keep] * (np.sqrt(lam[keep]) ** p) ** 2) @ Q[:, keep].T
     assert np.abs(out - exact).max() <= 1e-9 * max(1.0, np.abs(exact).max())
+
+
+def _pack_upper(A):
+    n = A.shape[0]
+    return np.concatenate([A[:j + 1, j] for j in range(n)]) if n else np.zeros(0)
+
+
+def _gen_pair(n, seed):
+    rng = np.random.default_rng(seed)
+    X = rng.standard_normal((n, n))
+    A = 0.5 * (X + X.T)
+    Y = rng.standard_normal((n, n)) / np.sqrt(n)
+    B = Y @ Y.T + 0.5 * np.eye(n)               # overlap-like SPD, cond ~ 10
+    return np.asfortranarray(A), np.asfortranarray(0.5 * (B + B.T))
+
+
+@pytest.mark.parametrize("n,num_orb", [(1, 1), (2, 1), (7, 3), (64, 64), (129, 40), (300, 100)])
+def test_dspgvx(D, n, num_orb):
+    print(f"[dspgvx] n={n} num_orb={num_orb}", flush=True)
+    A, B = _gen_pair(n, 0x5EED7600 + n)
+    Ap, Bp = _pack_upper(A), _pack_upper(B)
+    dA, dB = D.DeviceArray.from_host(Ap), D.DeviceArray.from_host(Bp)
+    dZ, dw = D.DeviceArray(n * num_orb), D.DeviceArray(n)
+    info = D.dspgvx(n, dA, dB, num_orb, dZ, dw)
+    assert info == 0
+    Z, w = dZ.to_host((n, num_orb)), dw.to_host((n,))
+    Zr, wr = M.dspgvx(Ap, Bp, n, num_orb)
+    an = np.abs(wr).max()
+    assert np.abs(w - wr).max() <= 1e-12 * n * an
+    assert np.all(w[num_orb:] == 0.0)
+    sgn = np.sign(np.sum(Z * (B @ Zr), axis=0))      # eigenvectors up to sign
+    assert np.all(sgn != 0)
+    assert np.abs(Z * sgn[None, :] - Zr).max() < 1e-9
+    assert np.abs(A @ Z - B @ Z * w[None, :num_orb]).max() <= 1e-12 * n * an
+    assert np.abs(Z.T @ B @ Z - np.eye(num_orb)).max() <= 1e-12 * n
+
+
+def test_dspgvx_overlap_not_pd(D):
+    """B's leading minor of order 3 is not positive definite: info = n + 3
+    (the reference panics with that minor)."""
+    n = 5
+    A = np.eye(n)
+    B = np.diag([1.0, 2.0, -1.0, 1.0, 1.0])
+    dZ, dw = D.DeviceArray(n * 2), D.DeviceArray(n)
+    info = D.dspgvx(n, D.DeviceArray.from_host(_pack_upper(A)),
+                    D.DeviceArray.from_host(_pack_upper(B)), 2, dZ, dw)
+    assert info == n + 3
+
+
+@pytest.mark.parametrize("n", [1, 3, 50, 200])
+def test_dspevx(D, n):
+    A, _ = _gen_pair(n, 0x5EED7700 + n)
+    Ap = _pack_upper(A)
+    dZ, dw = D.DeviceArray(n * n), D.DeviceArray(n)
+    nf = D.dspevx(n, D.DeviceArray.from_host(Ap), dZ, dw)
+    assert nf == n
+    Z, w = dZ.to_host((n, n)), dw.to_host((n,))
+    Zr, wr = M.dspevx(Ap, n)
+    an = np.abs(wr).max()
+    assert np.abs(w - wr).max() <= 1e-13 * n * an
+    sgn = np.sign(np.sum(Z * Zr, axis=0))
+    assert np.abs(Z * sgn[None, :] - Zr).max() < 1e-9
+
+
+def test_fock_scf_step(D):
+    """One closed-shell SCF step on the RI pipeline's own J and K:
+    F = H + J - K/2 packed, then F C = S C e with the overlap S."""
+    from oracle import ri_oracle as O
+    nbas, naux, nocc, seed = 48, 144, 6, 0x5EED7800
+    dA = D.DeviceArray(nbas * nbas * naux)
+    D.gen_ri3ao(dA, nbas, naux, seed=seed)
+    Cm = O.gen_orthonormal(nbas, seed=seed + 1)
+    dC = D.DeviceArray.from_host(Cm)
+    dD = D.DeviceArray(nbas * nbas)
+    D.density(dC, nbas, nocc, dD)
+    dJ = D.DeviceArray(nbas * nbas)
+    D.rij(dA, nbas, naux, dD, dJ)
+    dB = D.DeviceArray(nocc * nbas * naux)
+    D.ao2mo_occ(dC, nbas, nbas, nocc, dA, naux, dB)
+    dK = D.DeviceArray(nbas * nbas)
+    D.rik(dB, nocc, nbas, naux, dK, want_energy=False)
+    rng = np.random.default_rng(seed + 2)
+    X = rng.standard_normal((nbas, nbas))
+    H = np.asfortranarray(0.5 * (X + X.T))
+    dH = D.DeviceArray.from_host(H)
+    dF = D.DeviceArray(nbas * (nbas + 1) // 2)
+    D.fock_rhf_packed(nbas, dH, dJ, dK, dF)
+    J, K = dJ.to_host((nbas, nbas)), dK.to_host((nbas, nbas))
+    Fp = dF.to_host((nbas * (nbas + 1) // 2,))
+    assert np.array_equal(Fp, _pack_upper((H + J) - 0.5 * K))
+    _, S = _gen_pair(nbas, seed + 3)
+    Sp = _pack_upper(S)
+    dZ, dw = D.DeviceArray(nbas * nocc), D.DeviceArray(nbas)
+    assert D.dspgvx(nbas, dF, D.DeviceArray.from_host(Sp), nocc, dZ, dw) == 0
+    Zr, wr = M.dspgvx(Fp, Sp, nbas, nocc)
+    w = dw.to_host((nbas,))
+    assert np.abs(w[:nocc] - wr[:nocc]).max() <= 1e-11 * np.abs(wr).max()
