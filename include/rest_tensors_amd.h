@@ -248,6 +248,16 @@ int rt_upper_pack_dev(const double* full, int64_t n, double* packed,
  * pack per aux slice (rifull_to_matfull_symm, ri.rs:297-306, 330-341). */
 int rt_rifull_to_matfull_symm_dev(const double* ri, int64_t nao, int64_t naux,
                                   double* out, void* stream);
+/* special_dgemm_f_01 (restmatr.f90:111-154; the O_V V^-1/2 step of the ri3fn
+ * build) on device buffers, with the drop-in symbol's argument meaning: for
+ * every y, ten3_a[x-range, y, z-range] = alpha * ten3_a[x-range, y, z-range]
+ * * B[row-range, col-range] + beta * ten3_a[...], in place; the B block is
+ * len_z x len_z.  0-based starts. */
+int rt_special_dgemm_dev(double* ten3_a, int64_t x_a, int64_t y_a, int64_t z_a,
+                         int64_t start_x_a, int64_t len_x_a, int64_t start_z_a, int64_t len_z_a,
+                         const double* matr_b, int64_t rows_b, int64_t columns_b,
+                         int64_t start_row_b, int64_t len_row_b, int64_t start_column_b,
+                         int64_t len_column_b, double alpha, double beta, void* stream);
 /* Index permutations of a column-major [n0, n1, n2] RIFull (ri.rs:225-290):
  * order "jik" -> [n1, n0, n2] (transpose_jik), "jki" -> [n1, n2, n0]
  * (transpose_jki), "kji" -> [n2, n1, n0] (transpose_kji), "ikj" ->

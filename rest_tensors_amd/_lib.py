@@ -75,6 +75,8 @@ RT_SIGNATURES = {
     "rt_rifull_to_matfull_symm_dev": (i32, [vp, i64, i64, vp, vp]),
     "rt_upper_dim": (i64, [i64]),
     "rt_rifull_transpose_dev": (i32, [vp, i64, i64, i64, C.c_char_p, vp, vp]),
+    "rt_special_dgemm_dev": (i32, [vp, i64, i64, i64, i64, i64, i64, i64, vp, i64, i64, i64,
+                                   i64, i64, i64, f64, f64, vp]),
     "rt_copy_mm_dev": (i32, [i64, i64, vp, i64, i64, i64, i64, vp, i64, i64, i64, i64, vp]),
     "rt_copy_mr_dev": (i32, [i64, i64, vp, i64, i64, i64, i64, vp, i64, i64, i64, i64, i64,
                              i64, i32, vp]),

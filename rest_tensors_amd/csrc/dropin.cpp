@@ -136,40 +136,26 @@ static int general_dgemm_host(const double* A, int64_t ra, int64_t ca,
   return download_box(C, vc, dC);
 }
 
-static int special_dgemm_host(double* T, int64_t xa, int64_t ya, int64_t za,
-                              int64_t sxa, int64_t lxa, int64_t sza,
-                              int64_t lza, const double* Bm, int64_t rb,
-                              int64_t cb, int64_t srb, int64_t lrb,
-                              int64_t scb, int64_t lcb, double alpha,
-                              double beta) {
-  RTA_CHECK(xa >= 0 && ya >= 0 && za >= 0, "negative dimension");
-  RTA_CHECK(blk_ok(sxa, lxa, xa) && blk_ok(sza, lza, za) &&
-                blk_ok(srb, lrb, rb) && blk_ok(scb, lcb, cb),
-            "block outside its array");
-  RTA_CHECK(lrb == lza && lcb == lza,
-            "B block must be len_z x len_z (got %lldx%lld, len_z %lld)",
-            (long long)lrb, (long long)lcb, (long long)lza);
-  if (lxa == 0 || ya == 0 || lza == 0) return RT_OK;
-  RTA_CHECK(T && Bm, "NULL buffer");
-  // tensor box: the z-slab [sza, sza+lza) (contiguous), B box: its columns
-  BlockView vt{xa * ya, lza, 1, sza * xa * ya, 1, xa * ya, 0};
-  BlockView vb{rb, lcb, 1, scb * rb, 1, rb, 0};
-  double *dT, *dB, *dR;
-  RTA_TRY(upload_box(T, vt, SCRATCH_DROPIN_A, &dT));
-  RTA_TRY(upload_box(Bm, vb, SCRATCH_DROPIN_B, &dB));
-  RTA_TRY(scratch_get(SCRATCH_DROPIN_C, sizeof(double) * lxa * lza * ya,
-                      (void**)&dR));
-  // R_y = alpha * T[sxa:, y, z-slab] * B[srb:, :]   (out of place)
+// special_dgemm_f_01 on device buffers: Tz points at the z-slab start of the
+// xa x ya x za tensor (its element (x, y, z) at x + xa*y + xa*ya*(z - sza)), Bc
+// at column scb of the rb-row matrix B.  Per y slice the product is formed
+// out of place (R) and then combined: T = R + beta T.
+static int special_dgemm_core(double* Tz, int64_t xa, int64_t ya, int64_t sxa, int64_t lxa,
+                              int64_t lza, const double* Bc, int64_t rb, int64_t srb,
+                              double alpha, double beta, hipStream_t s) {
+  double* dR;
+  RTA_TRY(scratch_get(SCRATCH_DROPIN_C, sizeof(double) * lxa * lza * ya, (void**)&dR));
+  // R_y = alpha * T[sxa:, y, z-slab] * B[srb:, cols]
   GemmDesc d;
   d.ta = false;
   d.tb = false;
   d.M = lxa;
   d.N = lza;
   d.K = lza;
-  d.A = dT + sxa;
+  d.A = Tz + sxa;
   d.lda = xa * ya;
   d.sA = xa;
-  d.B = dB + srb;
+  d.B = Bc + srb;
   d.ldb = rb;
   d.sB = 0;
   d.C = dR;
@@ -179,10 +165,40 @@ static int special_dgemm_host(double* T, int64_t xa, int64_t ya, int64_t za,
   d.alpha = alpha;
   d.beta = 0.0;
   d.batch = ya;
-  RTA_TRY(gemm_batched(d, nullptr, "special_dgemm"));
+  RTA_TRY(gemm_batched(d, s, "special_dgemm"));
   // T[x, y, z] = R_y[x, z] + beta * T[x, y, z]
-  RTA_TRY(axpby3d(lxa, lza, ya, dR, 1, lxa, lxa * lza, dT + sxa, 1, xa * ya,
-                  xa, beta, nullptr));
+  return axpby3d(lxa, lza, ya, dR, 1, lxa, lxa * lza, Tz + sxa, 1, xa * ya, xa, beta, s);
+}
+
+static int special_dgemm_check(int64_t xa, int64_t ya, int64_t za, int64_t sxa, int64_t lxa,
+                               int64_t sza, int64_t lza, int64_t rb, int64_t cb, int64_t srb,
+                               int64_t lrb, int64_t scb, int64_t lcb) {
+  RTA_CHECK(xa >= 0 && ya >= 0 && za >= 0, "negative dimension");
+  RTA_CHECK(blk_ok(sxa, lxa, xa) && blk_ok(sza, lza, za) &&
+                blk_ok(srb, lrb, rb) && blk_ok(scb, lcb, cb),
+            "block outside its array");
+  RTA_CHECK(lrb == lza && lcb == lza,
+            "B block must be len_z x len_z (got %lldx%lld, len_z %lld)",
+            (long long)lrb, (long long)lcb, (long long)lza);
+  return RT_OK;
+}
+
+static int special_dgemm_host(double* T, int64_t xa, int64_t ya, int64_t za,
+                              int64_t sxa, int64_t lxa, int64_t sza,
+                              int64_t lza, const double* Bm, int64_t rb,
+                              int64_t cb, int64_t srb, int64_t lrb,
+                              int64_t scb, int64_t lcb, double alpha,
+                              double beta) {
+  RTA_TRY(special_dgemm_check(xa, ya, za, sxa, lxa, sza, lza, rb, cb, srb, lrb, scb, lcb));
+  if (lxa == 0 || ya == 0 || lza == 0) return RT_OK;
+  RTA_CHECK(T && Bm, "NULL buffer");
+  // tensor box: the z-slab [sza, sza+lza) (contiguous), B box: its columns
+  BlockView vt{xa * ya, lza, 1, sza * xa * ya, 1, xa * ya, 0};
+  BlockView vb{rb, lcb, 1, scb * rb, 1, rb, 0};
+  double *dT, *dB;
+  RTA_TRY(upload_box(T, vt, SCRATCH_DROPIN_A, &dT));
+  RTA_TRY(upload_box(Bm, vb, SCRATCH_DROPIN_B, &dB));
+  RTA_TRY(special_dgemm_core(dT, xa, ya, sxa, lxa, lza, dB, rb, srb, alpha, beta, nullptr));
   return download_box(T, vt, dT);
 }
 
@@ -368,3 +384,20 @@ void copy_rr_(const int* x_len, const int* y_len, const int* z_len,
 }
 
 }  // extern "C"
+
+// special_dgemm_f_01 (restmatr.f90:111-154) on device buffers, in place.
+extern "C" int rt_special_dgemm_dev(double* ten3_a, int64_t x_a, int64_t y_a, int64_t z_a,
+                                    int64_t start_x_a, int64_t len_x_a, int64_t start_z_a,
+                                    int64_t len_z_a, const double* matr_b, int64_t rows_b,
+                                    int64_t columns_b, int64_t start_row_b, int64_t len_row_b,
+                                    int64_t start_column_b, int64_t len_column_b, double alpha,
+                                    double beta, void* stream) {
+  RTA_TRY(special_dgemm_check(x_a, y_a, z_a, start_x_a, len_x_a, start_z_a, len_z_a, rows_b,
+                              columns_b, start_row_b, len_row_b, start_column_b, len_column_b));
+  if (len_x_a == 0 || y_a == 0 || len_z_a == 0) return RT_OK;
+  RTA_CHECK(ten3_a && matr_b, "NULL buffer");
+  return special_dgemm_core(ten3_a + start_z_a * x_a * y_a, x_a, y_a, start_x_a, len_x_a,
+                            len_z_a, matr_b + start_column_b * rows_b, rows_b, start_row_b,
+                            alpha, beta, (hipStream_t)stream);
+}
+

@@ -246,6 +246,13 @@ def matrix_power(n, A: DeviceArray, lda, p, threshold, out: DeviceArray, ldo, st
     check(lib().rt_matrix_power_dev(n, A.ptr, lda, p, threshold, out.ptr, ldo, stream))
 
 
+def special_dgemm(T: DeviceArray, xa, ya, za, sx, lx, sz, lz, B: DeviceArray, rb, cb, srb,
+                  lrb, scb, lcb, alpha, beta, stream=None):
+    """special_dgemm_f_01 semantics on device buffers (in place on T)."""
+    check(lib().rt_special_dgemm_dev(T.ptr, xa, ya, za, sx, lx, sz, lz, B.ptr, rb, cb, srb, lrb,
+                                     scb, lcb, alpha, beta, stream))
+
+
 def rifull_transpose(src: DeviceArray, n0, n1, n2, order: str, out: DeviceArray, stream=None):
     """RIFull::transpose_<order> of the column-major [n0, n1, n2] tensor."""
     check(lib().rt_rifull_transpose_dev(src.ptr, n0, n1, n2, order.encode(), out.ptr, stream))

@@ -210,6 +210,47 @@ def test_dropin_special_dgemm(D):
     assert relf(Tc, ref) < 1e-14
 
 
+@pytest.mark.parametrize("case", [(8, 5, 12, 2, 5, 3, 6, 10, 9, 1, 6, 2, 6, 0.5, -2.0),
+                                  (40, 33, 70, 0, 40, 0, 70, 70, 70, 0, 70, 0, 70, 1.0, 0.0),
+                                  (17, 3, 9, 4, 0, 2, 5, 5, 5, 0, 5, 0, 5, 1.0, 1.0)])
+def test_special_dgemm_dev(D, case):
+    """Device buffers, in place; same argument meaning as special_dgemm_f_01_."""
+    xa, ya, za, sx, lx, sz, lz, rb, cb, srb, lrb, scb, lcb, al, be = case
+    rng = np.random.default_rng(10)
+    T = np.asfortranarray(rng.standard_normal((xa, ya, za)))
+    Bm = np.asfortranarray(rng.standard_normal((rb, cb)))
+    ref = O.special_dgemm_01(T, sx, lx, sz, lz, Bm, srb, lrb, scb, lcb, al, be)
+    dT, dB = D.DeviceArray.from_host(T), D.DeviceArray.from_host(Bm)
+    D.special_dgemm(dT, xa, ya, za, sx, lx, sz, lz, dB, rb, cb, srb, lrb, scb, lcb, al, be)
+    assert relf(dT.to_host((xa, ya, za)), ref) < 1e-14
+
+
+def test_ri3fn_producer(D):
+    """ri3fn = O V^-1/2 (metric power, then the per-slice contraction), and
+    O M with V^-1 = M M^T (Cholesky route): both give ri3fn ri3fn^T = O V^-1 O^T."""
+    from oracle import metric_oracle as MO
+    nbas, naux = 24, 72
+    dO = D.DeviceArray(nbas * nbas * naux)
+    D.gen_ri3ao(dO, nbas, naux, seed=0x5EED7900)
+    Om = dO.to_host((nbas * nbas, naux))
+    V = MO.spd(naux, seed=0x5EED7901, lo=1e-4, hi=5.0)
+    dV, dM = D.DeviceArray.from_host(V), D.DeviceArray(naux * naux)
+    D.matrix_power(naux, dV, naux, -0.5, 1e-10, dM, naux)
+    D.special_dgemm(dO, nbas, nbas, naux, 0, nbas, 0, naux, dM, naux, naux, 0, naux, 0, naux,
+                    1.0, 0.0)
+    R = dO.to_host((nbas * nbas, naux))
+    ref = Om @ MO.lapack_power(V, -0.5, 1e-10)
+    assert relf(R, ref) < 1e-11
+    G = Om @ np.linalg.solve(V, Om.T)
+    assert relf(R @ R.T, G) < 1e-10
+    dO2, dL = D.DeviceArray.from_host(np.asfortranarray(Om)), D.DeviceArray(naux * naux)
+    D.cholesky_decompose_inverse("L", naux, dV, naux, dL, naux)
+    D.special_dgemm(dO2, nbas, nbas, naux, 0, nbas, 0, naux, dL, naux, naux, 0, naux, 0, naux,
+                    1.0, 0.0)
+    R2 = dO2.to_host((nbas * nbas, naux))
+    assert relf(R2 @ R2.T, G) < 1e-10
+
+
 # ---- device entry points -----------------------------------------------------------
 def test_gen_ri3ao_matches_host_generator(D):
     nbas, naux = 37, 5
