@@ -441,6 +441,11 @@ int rt_batch_create(const rt_mol_desc* mols, int nmol, void** handle);
 /* Times every candidate tile edge (64, 128) x persistent grid (1, 2, 4 per
  * CU) of each grouped GEMM stage on this molecule set and keeps the fastest. */
 int rt_batch_autotune(void* handle, int reps, void* stream);
+/* Fix one stage's tile (0 density, 1 ao2mo s1, 2 ao2mo s2, 3 K partials):
+ * bm x bn with waves_m of the 4 waves along M, grid_mul persistent
+ * workgroups per CU; the shape must be one the autotuner considers
+ * (RT_ERR_ARG otherwise).  Drops a captured graph. */
+int rt_batch_set_stage(void* handle, int stage, int bm, int bn, int waves_m, int grid_mul);
 /* Runs density, RI-J, ao2mo (occupied rows), RI-K and the energies of every
  * molecule (8 launches). */
 int rt_batch_run(void* handle, void* stream);

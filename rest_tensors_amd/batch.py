@@ -32,6 +32,12 @@ class Batch:
     def autotune(self, reps=2, stream=None):
         check(lib().rt_batch_autotune(self.handle, reps, stream))
 
+    STAGES = {"density": 0, "ao2mo_s1": 1, "ao2mo_s2": 2, "k_partials": 3}
+
+    def set_tiles(self, stage: str, bm: int, bn: int, waves_m: int, grid_mul: int = 2):
+        check(lib().rt_batch_set_stage(self.handle, self.STAGES[stage], bm, bn, waves_m,
+                                       grid_mul))
+
     def run(self, stream=None):
         check(lib().rt_batch_run(self.handle, stream))
 

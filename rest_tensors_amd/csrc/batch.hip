@@ -158,9 +158,21 @@ __global__ __launch_bounds__(256) void batch_energy_kernel(const MolDev* __restr
 
 // ---- host side ------------------------------------------------------------------
 struct GroupCfg {
-  int tile = 64;   // 64 or 128
-  int grid_mul = 2;  // persistent workgroups per CU
+  int bm = 64, bn = 64;  // tile; the upper-tile K stage uses square tiles
+  int wm = 2;            // waves along M (4 / wm along N)
+  int grid_mul = 2;      // persistent workgroups per CU
 };
+
+// Tile candidates per stage.  Stages 1 and 2 have one side nocc = nbas/8 wide
+// (8..48 for the grouped workload): narrow tiles on that side, the 4 waves
+// stacked along the wide one.
+static std::vector<GroupCfg> stage_candidates(int s) {
+  if (s == 1) return {{64, 64, 2}, {128, 128, 2}, {128, 32, 4}, {128, 48, 4}, {256, 32, 4},
+                      {256, 48, 4}};
+  if (s == 2) return {{64, 64, 2}, {128, 128, 2}, {32, 128, 1}, {48, 128, 1}, {32, 256, 1},
+                      {48, 256, 1}};
+  return {{64, 64, 2}, {128, 128, 2}};
+}
 
 struct GroupedStage {
   bool ta = true, tb = false;
@@ -195,7 +207,7 @@ static int dalloc(Batch* b, size_t bytes, void** p) {
   return RT_OK;
 }
 
-static GemmParams make_params(const GemmDesc& d, int tile, bool reduce, int64_t bchunk) {
+static GemmParams make_params(const GemmDesc& d, int bm, int bn, bool reduce, int64_t bchunk) {
   GemmParams g{};
   g.A = d.A;
   g.B = d.B;
@@ -213,8 +225,8 @@ static GemmParams make_params(const GemmDesc& d, int tile, bool reduce, int64_t 
   g.alpha = d.alpha;
   g.beta = d.beta;
   g.nbatch = d.batch;
-  g.tiles_m = (int)((d.M + tile - 1) / tile);
-  g.tiles_n = (int)((d.N + tile - 1) / tile);
+  g.tiles_m = (int)((d.M + bm - 1) / bm);
+  g.tiles_n = (int)((d.N + bn - 1) / bn);
   if (reduce) {
     g.bchunk = bchunk;
     g.nchunks = (d.batch + bchunk - 1) / bchunk;
@@ -230,9 +242,10 @@ static GemmParams make_params(const GemmDesc& d, int tile, bool reduce, int64_t 
   return g;
 }
 
-// Descriptors of stage s for tile edge `tile`.  For the K stage the batch
+// Descriptors of stage s for tile configuration c.  For the K stage the batch
 // chunking is fixed by the partial-slot layout chosen at creation.
-static void build_stage(Batch* b, int s, int tile) {
+static void build_stage(Batch* b, int s, const GroupCfg& c) {
+  const int bm = c.bm, bn = c.bn;
   GroupedStage& S = b->st[s];
   S.descs.clear();
   for (const MolDev& m : b->mols) {
@@ -249,7 +262,7 @@ static void build_stage(Batch* b, int s, int tile) {
       d.ldb = m.nb;
       d.C = m.D;
       d.c_cs = m.nb;
-      S.descs.push_back(make_params(d, tile, false, 1));
+      S.descs.push_back(make_params(d, bm, bn, false, 1));
     } else if (s == 1) {  // X^T_P = A_P^T C_occ
       d.M = m.nb;
       d.N = m.nocc;
@@ -263,7 +276,7 @@ static void build_stage(Batch* b, int s, int tile) {
       d.c_cs = m.nb;
       d.sC = m.nb * m.nocc;
       d.batch = m.naux;
-      S.descs.push_back(make_params(d, tile, false, 1));
+      S.descs.push_back(make_params(d, bm, bn, false, 1));
     } else if (s == 2) {  // B_P = (X^T_P)^T C
       d.M = m.nocc;
       d.N = m.nb;
@@ -277,7 +290,7 @@ static void build_stage(Batch* b, int s, int tile) {
       d.c_cs = m.nocc;
       d.sC = m.nocc * m.nb;
       d.batch = m.naux;
-      S.descs.push_back(make_params(d, tile, false, 1));
+      S.descs.push_back(make_params(d, bm, bn, false, 1));
     } else {  // K partials: sum_P B_P^T B_P, upper tiles
       d.M = d.N = m.nb;
       d.K = m.nocc;
@@ -293,11 +306,13 @@ static void build_stage(Batch* b, int s, int tile) {
       d.sC = m.nb * m.nb;
       d.batch = m.naux;
       const int64_t bchunk = (m.naux + m.nchunks - 1) / m.nchunks;
-      GemmParams g = make_params(d, tile, true, bchunk);
+      GemmParams g = make_params(d, bm, bn, true, bchunk);
       S.descs.push_back(g);
     }
   }
-  S.cfg.tile = tile;
+  S.cfg.bm = bm;
+  S.cfg.bn = bn;
+  S.cfg.wm = c.wm;
   S.ta = (s != 0);
   S.tb = (s == 0);
 }
@@ -318,23 +333,39 @@ static int upload_stage(Batch* b, int s) {
   return RT_OK;
 }
 
-template <int T, bool TA, bool TB>
+template <int BM, int BN, int WM, bool TA, bool TB>
 static void launch_grouped_t(const GroupedStage& S, unsigned grid, hipStream_t s) {
-  hipLaunchKernelGGL((dgemm_grouped_kernel<T, T, 16, TA, TB>), dim3(grid), dim3(256), 0,
+  hipLaunchKernelGGL((dgemm_grouped_kernel<BM, BN, 16, TA, TB, WM>), dim3(grid), dim3(256), 0,
                      s, S.d_descs, S.d_offs, (int)S.descs.size());
+}
+
+template <bool TA, bool TB>
+static int launch_grouped_cfg(const GroupedStage& S, unsigned grid, hipStream_t s) {
+  const GroupCfg& c = S.cfg;
+#define RTA_G(BM, BN, WM) \
+  if (c.bm == BM && c.bn == BN && c.wm == WM) { launch_grouped_t<BM, BN, WM, TA, TB>(S, grid, s); return RT_OK; }
+  RTA_G(64, 64, 2)
+  RTA_G(128, 128, 2)
+  if constexpr (TA && !TB) {
+    RTA_G(128, 32, 4)
+    RTA_G(128, 48, 4)
+    RTA_G(256, 32, 4)
+    RTA_G(256, 48, 4)
+    RTA_G(32, 128, 1)
+    RTA_G(48, 128, 1)
+    RTA_G(32, 256, 1)
+    RTA_G(48, 256, 1)
+  }
+#undef RTA_G
+  return fail(RT_ERR_ARG, "batch: no kernel for tile %dx%d (%d waves along M)", c.bm, c.bn, c.wm);
 }
 
 static int launch_grouped(const GroupedStage& S, hipStream_t s) {
   const int64_t want = (int64_t)device_cu_count() * S.cfg.grid_mul;
   const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(want, S.total));
   if (S.total == 0) return RT_OK;
-  if (S.cfg.tile == 128) {
-    if (S.ta) launch_grouped_t<128, true, false>(S, grid, s);
-    else launch_grouped_t<128, false, true>(S, grid, s);
-  } else {
-    if (S.ta) launch_grouped_t<64, true, false>(S, grid, s);
-    else launch_grouped_t<64, false, true>(S, grid, s);
-  }
+  if (S.ta) RTA_TRY((launch_grouped_cfg<true, false>(S, grid, s)));
+  else RTA_TRY((launch_grouped_cfg<false, true>(S, grid, s)));
   RTA_HIP(hipGetLastError());
   return RT_OK;
 }
@@ -440,7 +471,7 @@ extern "C" int rt_batch_create(const rt_mol_desc* mols, int nmol, void** handle)
                 hipMemcpyHostToDevice) != hipSuccess)
     return fail_cleanup(fail(RT_ERR_HIP, "batch: descriptor upload failed"));
   for (int s = 0; s < 4; ++s) {
-    build_stage(b, s, 64);
+    build_stage(b, s, GroupCfg{});
     st = upload_stage(b, s);
     if (st != RT_OK) return fail_cleanup(st);
   }
@@ -460,8 +491,8 @@ extern "C" int rt_batch_autotune(void* h, int reps, void* stream) {
   for (int stg = 0; stg < 4; ++stg) {
     double best = 1e30;
     GroupCfg bestc;
-    for (int tile : {64, 128}) {
-      build_stage(b, stg, tile);
+    for (const GroupCfg& cand : stage_candidates(stg)) {
+      build_stage(b, stg, cand);
       RTA_TRY(upload_stage(b, stg));
       for (int gm : {1, 2, 4}) {
         b->st[stg].cfg.grid_mul = gm;
@@ -472,12 +503,38 @@ extern "C" int rt_batch_autotune(void* h, int reps, void* stream) {
         }
       }
     }
-    build_stage(b, stg, bestc.tile);
+    build_stage(b, stg, bestc);
     RTA_TRY(upload_stage(b, stg));
     b->st[stg].cfg = bestc;
     b->tune_ms[stg] = best;
   }
   if (b->exec) {  // the graph baked in the old descriptors' launch shapes
+    (void)hipGraphExecDestroy(b->exec);
+    (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+  }
+  return RT_OK;
+}
+
+extern "C" int rt_batch_set_stage(void* h, int stage, int bm, int bn, int waves_m,
+                                  int grid_mul) {
+  RTA_CHECK(h, "NULL handle");
+  RTA_CHECK(stage >= 0 && stage < 4, "stage must be 0..3 (density, ao2mo s1, ao2mo s2, K)");
+  RTA_CHECK(grid_mul >= 1 && grid_mul <= 64, "grid_mul must be 1..64");
+  Batch* b = (Batch*)h;
+  bool known = false;
+  for (const GroupCfg& c : stage_candidates(stage))
+    known = known || (c.bm == bm && c.bn == bn && c.wm == waves_m);
+  RTA_CHECK(known, "stage %d has no %dx%d tile with %d waves along M", stage, bm, bn, waves_m);
+  GroupCfg c;
+  c.bm = bm;
+  c.bn = bn;
+  c.wm = waves_m;
+  build_stage(b, stage, c);
+  RTA_TRY(upload_stage(b, stage));
+  b->st[stage].cfg.grid_mul = grid_mul;
+  if (b->exec) {  // a captured graph holds the old launch shapes
     (void)hipGraphExecDestroy(b->exec);
     (void)hipGraphDestroy(b->graph);
     b->exec = nullptr;
@@ -548,9 +605,11 @@ extern "C" int rt_batch_info(void* h, char* buf, int len) {
   std::string s = "{";
   for (int i = 0; i < 4; ++i) {
     char t[160];
-    snprintf(t, sizeof t, "%s\"%s\": {\"tile\": %d, \"grid_mul\": %d, \"tuned_ms\": %.4f}",
-             i ? ", " : "", names[i], b->st[i].cfg.tile, b->st[i].cfg.grid_mul,
-             b->tune_ms[i]);
+    snprintf(t, sizeof t,
+             "%s\"%s\": {\"tile\": [%d, %d], \"waves_m\": %d, \"grid_mul\": %d, "
+             "\"tuned_ms\": %.4f}",
+             i ? ", " : "", names[i], b->st[i].cfg.bm, b->st[i].cfg.bn, b->st[i].cfg.wm,
+             b->st[i].cfg.grid_mul, b->tune_ms[i]);
     s += t;
   }
   s += "}";

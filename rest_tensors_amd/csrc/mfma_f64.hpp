@@ -76,6 +76,7 @@ __device__ __forceinline__ void load_tile(const double* __restrict__ P,
                                           int64_t ld, int64_t r0, int64_t rmax,
                                           int64_t k0, int64_t kmax,
                                           double (&reg)[R * BK / 256]) {
+  static_assert(KCONTIG || 256 % R == 0, "row-contiguous tiles need R | 256");
   const int t = threadIdx.x;
 #pragma unroll
   for (int p = 0; p < R * BK / 256; ++p) {
@@ -118,13 +119,17 @@ constexpr int kGemmPad = 1;
 // One work item w (tile x batch chunk) of a GEMM; every thread of the 256-wide
 // workgroup must call it (it synchronizes).  As/Bs: LDS of
 // BK*(BM+1) and BK*(BN+1) doubles.
-template <int BM, int BN, int BK, bool TA, bool TB>
+template <int BM, int BN, int BK, bool TA, bool TB, int WAVES_M = 2>
 __device__ __forceinline__ void dgemm_tile(const GemmParams& g, int64_t w,
                                            double* __restrict__ As,
                                            double* __restrict__ Bs) {
   constexpr int PAD = kGemmPad;
-  constexpr int WM = BM / 2, WN = BN / 2;
+  // 4 waves as WAVES_M x (4 / WAVES_M): narrow tiles (one side 32 or 48 wide)
+  // stack the waves along the wide side
+  constexpr int WAVES_N = 4 / WAVES_M;
+  constexpr int WM = BM / WAVES_M, WN = BN / WAVES_N;
   constexpr int FM = WM / 16, FN = WN / 16;
+  static_assert(WAVES_M * WAVES_N == 4 && WM % 16 == 0 && WN % 16 == 0, "wave layout");
   static_assert(BK % 4 == 0 && (BM * BK) % 256 == 0 && (BN * BK) % 256 == 0,
                 "tile shape");
   const int64_t chunk = w / g.ntiles;
@@ -144,7 +149,7 @@ __device__ __forceinline__ void dgemm_tile(const GemmParams& g, int64_t w,
 
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
-  const int wave_m = wave & 1, wave_n = wave >> 1;
+  const int wave_m = wave % WAVES_M, wave_n = wave / WAVES_M;
 
   f64x4 acc[FM][FN];
 #pragma unroll
@@ -225,7 +230,7 @@ __global__ __launch_bounds__(256) void dgemm_mfma_kernel(GemmParams g) {
 // Persistent grouped GEMM: descs[d] is one (batched or batch-reducing) GEMM
 // with descs[d].nwork work items; offs holds the exclusive prefix sums of
 // nwork (offs[ndesc] = total).  Workgroups stride over the global work list.
-template <int BM, int BN, int BK, bool TA, bool TB>
+template <int BM, int BN, int BK, bool TA, bool TB, int WAVES_M = 2>
 __global__ __launch_bounds__(256) void dgemm_grouped_kernel(
     const GemmParams* __restrict__ descs, const int64_t* __restrict__ offs,
     int ndesc) {
@@ -240,7 +245,7 @@ __global__ __launch_bounds__(256) void dgemm_grouped_kernel(
       else hi = mid;
     }
     const GemmParams g = descs[lo];
-    dgemm_tile<BM, BN, BK, TA, TB>(g, gw - offs[lo], As, Bs);
+    dgemm_tile<BM, BN, BK, TA, TB, WAVES_M>(g, gw - offs[lo], As, Bs);
     __syncthreads();  // LDS is reused by the next work item
   }
 }

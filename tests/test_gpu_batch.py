@@ -82,7 +82,8 @@ def test_batch_eager_autotune_and_graph(D):
     b.autotune(reps=1)
     info = b.info()
     assert set(info) == {"density", "ao2mo_s1", "ao2mo_s2", "k_partials"}
-    assert all(v["tile"] in (64, 128) for v in info.values())
+    assert all(len(v["tile"]) == 2 and v["waves_m"] in (1, 2, 4) for v in info.values())
+    assert info["k_partials"]["tile"][0] == info["k_partials"]["tile"][1]   # upper tiles
     b.run()
     ej, ex = b.energies()
     check_against(mols, ej, ex, refs)
@@ -102,3 +103,18 @@ def test_batch_eager_autotune_and_graph(D):
         check_against(mols, ej, ex, refs)
         single(D, mols[rep])   # unrelated eager work between replays
     s.destroy()
+
+
+@pytest.mark.parametrize("s1,s2", [((128, 48, 4), (48, 128, 1)), ((256, 32, 4), (32, 256, 1))])
+def test_batch_narrow_tiles(D, s1, s2):
+    """The narrow ao2mo tiles (waves stacked along the wide side), forced,
+    against the single-molecule path; nbas not a multiple of 16 or 64."""
+    from rest_tensors_amd.batch import Batch
+    mols = make_mols(D, [67, 130, 99, 250], seed0=4200)
+    refs = [single(D, m) for m in mols]
+    b = Batch(mols)
+    b.set_tiles("ao2mo_s1", *s1)
+    b.set_tiles("ao2mo_s2", *s2)
+    b.run()
+    ej, ex = b.energies()
+    check_against(mols, ej, ex, refs)
