@@ -441,6 +441,9 @@ int rt_batch_create(const rt_mol_desc* mols, int nmol, void** handle);
 /* Times every candidate tile edge (64, 128) x persistent grid (1, 2, 4 per
  * CU) of each grouped GEMM stage on this molecule set and keeps the fastest. */
 int rt_batch_autotune(void* handle, int reps, void* stream);
+/* Refresh the batch's padded copy of every molecule's ri3ao (made at
+ * rt_batch_create) after the caller changed ri3ao; ordered on `stream`. */
+int rt_batch_reload(void* handle, void* stream);
 /* Fix one stage's tile (0 density, 1 ao2mo s1, 2 ao2mo s2, 3 K partials):
  * bm x bn with waves_m of the 4 waves along M, grid_mul persistent
  * workgroups per CU; the shape must be one the autotuner considers

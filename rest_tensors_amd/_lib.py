@@ -110,6 +110,7 @@ RT_SIGNATURES = {
     "rt_batch_create": (i32, [vp, i32, C.POINTER(vp)]),
     "rt_batch_autotune": (i32, [vp, i32, vp]),
     "rt_batch_set_stage": (i32, [vp, i32, i32, i32, i32, i32]),
+    "rt_batch_reload": (i32, [vp, vp]),
     "rt_batch_run": (i32, [vp, vp]),
     "rt_batch_graph_capture": (i32, [vp, vp]),
     "rt_batch_graph_launch": (i32, [vp, vp]),

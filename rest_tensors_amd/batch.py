@@ -38,6 +38,10 @@ class Batch:
         check(lib().rt_batch_set_stage(self.handle, self.STAGES[stage], bm, bn, waves_m,
                                        grid_mul))
 
+    def reload(self, stream=None):
+        """Refresh the padded ri3ao copies after the molecules' ri3ao changed."""
+        check(lib().rt_batch_reload(self.handle, stream))
+
     def run(self, stream=None):
         check(lib().rt_batch_run(self.handle, stream))
 

@@ -5,9 +5,25 @@
 //   1 density      D_m = C_occ C_occ^T          grouped GEMM (persistent)
 //   2 gamma        gamma_m,P = <A_m,P, D_m>      grouped, one block per (m, P)
 //   3 J            J_m = sum_P A_m,P gamma_m,P   grouped, one block per (m, j, seg)
-//   4 ao2mo s1     X^T = A_P^T C_occ             grouped GEMM
-//   5 ao2mo s2     B_P = (X^T)^T C               grouped GEMM
-//   6 K partials   sum_P B_P^T B_P (upper tiles) grouped GEMM, batch-reducing
+//   4 ao2mo s1     X^T = A_P^T C_occ             grouped TN GEMM (direct-to-LDS)
+//   5 ao2mo s2     B_P = (X^T)^T C               grouped TN GEMM
+//   6 K partials   sum_P B_P^T B_P (upper tiles) grouped TN GEMM, batch-reducing
+// Stages 4-6 run on the TN engine, which needs k-contiguous operands with a
+// depth and leading dimension that are multiples of 16 and 16-byte aligned
+// rows.  The batch keeps its own zero-padded layouts (nbp, noccp = nb, nocc
+// rounded up to 16): ri3ao as Ap [nbp, nbp, naux] (copied once at creation,
+// again by rt_batch_reload), C as Cp [nbp, nb], X^T as [nbp, naux, noccp]
+// (rows (nu, P) = nu + nbp P) and B as Bp [naux, noccp, nb] (rows (P, o) =
+// P + naux o).  Each stage is then one GEMM per molecule with the aux index
+// inside the row index:
+//   4  X^T[(nu,P), o]  = sum_mu Ap[mu, (nu,P)] Cp[mu, o]     M = nbp naux
+//   5  Bp[(P,o), i]    = sum_nu X^T[nu, (P,o)] Cp[nu, i]      M = naux noccp
+//   6  K slots         = sum over (P,o) of Bp[(P,o), i] Bp[(P,o), j], the
+//                        depth (P,o) contiguous, split into 16-deep batches
+//                        that the partial slots sum in order.
+// The caller's B ([nocc, nb, naux]) is filled from Bp by a transposing copy.
+// RTA_BATCH_GROUPED=1 keeps stages 4-6 on the generic grouped engine (no
+// padded ri3ao; X^T per slice as [nbp, noccp]).
 //   7 K reduce     slot sums + mirror            grouped, one block per (m, 32-tile)
 //   8 energies     E_J, E_x per molecule         one block per molecule
 // The eight launches are captured once into a HIP graph and replayed.  The
@@ -25,6 +41,7 @@
 #include "common.hpp"
 #include "gemm.hpp"
 #include "mfma_f64.hpp"
+#include "mfma_tn.hpp"
 #include "timer.hpp"
 
 namespace rta {
@@ -34,8 +51,12 @@ struct MolDev {
   const double* C;
   double *D, *J, *K, *B;
   double *X, *gamma, *part;
+  double *Bp, *Cp;  // padded B [naux, noccp, nb] and C [nbp, nb] (X^T: see above)
+  double* Ap;       // padded ri3ao [nbp, nbp, naux] (TN path only)
   int64_t nb, naux, nocc;
+  int64_t nbp, noccp;
   int64_t nchunks;  // K partial slots
+  int64_t bchunk;   // per slot: aux slices (generic path) or 16-deep batches (TN path)
 };
 
 __device__ __forceinline__ int find_mol(const int64_t* __restrict__ offs, int nmol,
@@ -135,6 +156,68 @@ __global__ __launch_bounds__(256) void batch_kreduce_kernel(
 
 // one block per molecule: energies[2m] = E_J = 1/2 sum gamma^2,
 // energies[2m+1] = E_x = -1/2 sum_{n<nocc} K[n,n]
+// Cp(nu, i) = C(nu, i), nu < nb; one block per (molecule, column i).  Rows
+// nb..nbp of Cp are zero from creation and never written.
+__global__ __launch_bounds__(256) void batch_cpad_kernel(const MolDev* __restrict__ mols,
+                                                         const int64_t* __restrict__ offs,
+                                                         int nmol) {
+  const int m = find_mol(offs, nmol, blockIdx.x);
+  const MolDev M = mols[m];
+  const int64_t i = blockIdx.x - offs[m];
+  for (int64_t nu = threadIdx.x; nu < M.nb; nu += 256) M.Cp[nu + M.nbp * i] = M.C[nu + M.nb * i];
+}
+
+// Ap[mu, nu, P] = A[mu, nu, P] (zero outside nb x nb); one block per (molecule, P)
+__global__ __launch_bounds__(256) void batch_apad_kernel(const MolDev* __restrict__ mols,
+                                                         const int64_t* __restrict__ offs,
+                                                         int nmol) {
+  const int m = find_mol(offs, nmol, blockIdx.x);
+  const MolDev M = mols[m];
+  const int64_t P = blockIdx.x - offs[m];
+  const int64_t nb = M.nb, nbp = M.nbp;
+  const double* src = M.A + P * nb * nb;
+  double* dst = M.Ap + P * nbp * nbp;
+  for (int64_t e = threadIdx.x; e < nbp * nbp; e += 256) {
+    const int64_t mu = e % nbp, nu = e / nbp;
+    dst[e] = (mu < nb && nu < nb) ? src[mu + nb * nu] : 0.0;
+  }
+}
+
+// B[o, i, P] = Bp[(P, o), i] for o < nocc: per column i a transpose between
+// P-fastest (Bp) and o-fastest (B) orders.  One block per (molecule, column
+// i, 32 aux slices): the 32 x nocc block goes through LDS, read along P and
+// written along o (nocc <= kBcopyMaxOcc; larger nocc takes several passes).
+constexpr int kBcopyMaxOcc = 64;
+
+__global__ __launch_bounds__(256) void batch_bcopy_kernel(const MolDev* __restrict__ mols,
+                                                          const int64_t* __restrict__ offs,
+                                                          int nmol) {
+  __shared__ double T[kBcopyMaxOcc][33];
+  const int m = find_mol(offs, nmol, blockIdx.x);
+  const MolDev M = mols[m];
+  const int64_t loc = blockIdx.x - offs[m];
+  const int64_t npt = (M.naux + 31) / 32;
+  const int64_t i = loc / npt, P0 = (loc % npt) * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+  const double* src = M.Bp + M.naux * M.noccp * i;        // (P, o) at P + naux o
+  for (int64_t o0 = 0; o0 < M.nocc; o0 += kBcopyMaxOcc) {
+    const int64_t oc = (M.nocc - o0 < kBcopyMaxOcc) ? M.nocc - o0 : kBcopyMaxOcc;
+    __syncthreads();
+    for (int64_t o = ty; o < oc; o += 8) {
+      const int64_t P = P0 + tx;
+      if (P < M.naux) T[o][tx] = src[P + M.naux * (o0 + o)];
+    }
+    __syncthreads();
+    // write: for each P of the block, oc consecutive o (lanes along o)
+    for (int idx = threadIdx.x; idx < 32 * oc; idx += 256) {
+      const int pp = (int)(idx / oc);
+      const int64_t o = idx % oc;
+      const int64_t P = P0 + pp;
+      if (P < M.naux) M.B[(o0 + o) + M.nocc * i + M.nocc * M.nb * P] = T[o][pp];
+    }
+  }
+}
+
 __global__ __launch_bounds__(256) void batch_energy_kernel(const MolDev* __restrict__ mols,
                                                            double* __restrict__ energies) {
   __shared__ double red[2][4];
@@ -183,6 +266,21 @@ struct GroupedStage {
   int64_t total = 0;
 };
 
+// Stages 5-6 on the grouped TN engine (NSTAGE 2, BK 16)
+struct TnStage {
+  GroupCfg cfg;
+  std::vector<TnParams> descs;
+  TnParams* d_descs = nullptr;
+  int64_t* d_offs = nullptr;
+  int64_t total = 0;
+};
+
+static std::vector<GroupCfg> tn_candidates(int s) {
+  if (s == 1) return {{128, 64, 2}, {128, 32, 4}, {256, 32, 4}, {256, 64, 4}};
+  if (s == 2) return {{128, 128, 2}, {64, 128, 2}, {64, 64, 2}, {32, 128, 1}};
+  return {{64, 64, 2}, {128, 128, 2}};
+}
+
 struct Batch {
   int nmol = 0;
   std::vector<MolDev> mols;
@@ -192,6 +290,12 @@ struct Batch {
   double* energies = nullptr;  // device [2*nmol]
   std::vector<void*> owned;    // device allocations of the handle
   GroupedStage st[4];          // density, ao2mo s1, ao2mo s2, K partials
+  TnStage tn[4];               // TN forms of stages 2 and 3 (used when use_tn)
+  bool use_tn = true;
+  int64_t* d_off_cpad = nullptr;  // (molecule, column) offsets for the C padding
+  int64_t n_cpad = 0;
+  int64_t* d_off_bcopy = nullptr;  // (molecule, column, 32 slices) offsets for the B copy
+  int64_t n_bcopy = 0;
   double tune_ms[4] = {0, 0, 0, 0};
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -272,9 +376,9 @@ static void build_stage(Batch* b, int s, const GroupCfg& c) {
       d.sA = m.nb * m.nb;
       d.B = m.C;
       d.ldb = m.nb;
-      d.C = m.X;
-      d.c_cs = m.nb;
-      d.sC = m.nb * m.nocc;
+      d.C = m.X;  // padded [nbp, noccp] per slice
+      d.c_cs = m.nbp;
+      d.sC = m.nbp * m.noccp;
       d.batch = m.naux;
       S.descs.push_back(make_params(d, bm, bn, false, 1));
     } else if (s == 2) {  // B_P = (X^T_P)^T C
@@ -282,8 +386,8 @@ static void build_stage(Batch* b, int s, const GroupCfg& c) {
       d.N = m.nb;
       d.K = m.nb;
       d.A = m.X;
-      d.lda = m.nb;
-      d.sA = m.nb * m.nocc;
+      d.lda = m.nbp;
+      d.sA = m.nbp * m.noccp;
       d.B = m.C;
       d.ldb = m.nb;
       d.C = m.B;
@@ -305,8 +409,7 @@ static void build_stage(Batch* b, int s, const GroupCfg& c) {
       d.c_cs = m.nb;
       d.sC = m.nb * m.nb;
       d.batch = m.naux;
-      const int64_t bchunk = (m.naux + m.nchunks - 1) / m.nchunks;
-      GemmParams g = make_params(d, bm, bn, true, bchunk);
+      GemmParams g = make_params(d, bm, bn, true, m.bchunk);
       S.descs.push_back(g);
     }
   }
@@ -315,6 +418,115 @@ static void build_stage(Batch* b, int s, const GroupCfg& c) {
   S.cfg.wm = c.wm;
   S.ta = (s != 0);
   S.tb = (s == 0);
+}
+
+static void build_tn_stage(Batch* b, int s, const GroupCfg& c) {
+  TnStage& S = b->tn[s];
+  S.descs.clear();
+  for (const MolDev& m : b->mols) {
+    TnParams p{};
+    p.alpha = 1.0;
+    p.beta = 0.0;
+    p.nbatch = 1;
+    p.bchunk = 1;
+    p.nchunks = 1;
+    if (s == 1) {  // X^T[(nu,P), o] = sum_mu Ap[mu, (nu,P)] Cp[mu, o], o < nocc
+      p.A = m.Ap;
+      p.lda = m.nbp;
+      p.B = m.Cp;
+      p.ldb = m.nbp;
+      p.C = m.X;
+      p.c_rs = 1;
+      p.c_cs = m.nbp * m.naux;
+      p.M = m.nbp * m.naux;
+      p.N = m.nocc;  // columns nocc..noccp of X^T stay zero
+      p.K = m.nbp;
+    } else if (s == 2) {  // Bp[(P,o), i] = sum_nu X^T[nu, (P,o)] Cp[nu, i]
+      p.A = m.X;          // row (P, o) = P + naux o at nbp (P + naux o)
+      p.lda = m.nbp;
+      p.B = m.Cp;
+      p.ldb = m.nbp;
+      p.C = m.Bp;
+      p.c_rs = 1;
+      p.c_cs = m.naux * m.noccp;
+      p.M = m.naux * m.noccp;
+      p.N = m.nb;
+      p.K = m.nbp;
+    } else {  // K slots: depth (P,o) of each row of Bp, 16-deep batches
+      p.A = m.Bp;
+      p.B = m.Bp;
+      p.lda = p.ldb = m.naux * m.noccp;
+      p.sA = p.sB = 16;
+      p.C = m.part;
+      p.c_rs = 1;
+      p.c_cs = m.nb;
+      p.sC = m.nb * m.nb;
+      p.M = p.N = m.nb;
+      p.K = 16;
+      p.nbatch = m.naux * m.noccp / 16;
+      p.bchunk = m.bchunk;
+      p.nchunks = m.nchunks;
+      p.reduce = 1;
+      p.upper = 1;
+    }
+    p.tiles_m = (int)((p.M + c.bm - 1) / c.bm);
+    p.tiles_n = (int)((p.N + c.bn - 1) / c.bn);
+    p.ntiles = p.upper ? (int64_t)p.tiles_m * (p.tiles_m + 1) / 2
+                       : (int64_t)p.tiles_m * p.tiles_n;
+    p.nwork = p.ntiles * p.nchunks;
+    S.descs.push_back(p);
+  }
+  S.cfg.bm = c.bm;
+  S.cfg.bn = c.bn;
+  S.cfg.wm = c.wm;
+}
+
+static int upload_tn_stage(Batch* b, int s) {
+  TnStage& S = b->tn[s];
+  std::vector<int64_t> offs(S.descs.size() + 1, 0);
+  for (size_t i = 0; i < S.descs.size(); ++i) offs[i + 1] = offs[i] + S.descs[i].nwork;
+  S.total = offs.back();
+  if (!S.d_descs) {
+    RTA_TRY(dalloc(b, sizeof(TnParams) * S.descs.size(), (void**)&S.d_descs));
+    RTA_TRY(dalloc(b, sizeof(int64_t) * offs.size(), (void**)&S.d_offs));
+  }
+  RTA_HIP(hipMemcpy(S.d_descs, S.descs.data(), sizeof(TnParams) * S.descs.size(),
+                    hipMemcpyHostToDevice));
+  RTA_HIP(hipMemcpy(S.d_offs, offs.data(), sizeof(int64_t) * offs.size(),
+                    hipMemcpyHostToDevice));
+  return RT_OK;
+}
+
+template <int BM, int BN, int WM>
+static void launch_tn_t(const TnStage& S, unsigned grid, hipStream_t s) {
+  constexpr size_t lds = 2 * (BM + BN) * 16 * sizeof(double);
+  static bool attr = false;
+  if (!attr) {
+    (void)hipFuncSetAttribute((const void*)dgemm_grouped_tn_kernel<BM, BN, 16, 2, WM>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    attr = true;
+  }
+  hipLaunchKernelGGL((dgemm_grouped_tn_kernel<BM, BN, 16, 2, WM>), dim3(grid), dim3(256), lds, s,
+                     S.d_descs, S.d_offs, (int)S.descs.size());
+}
+
+static int launch_tn(const TnStage& S, hipStream_t s) {
+  if (S.total == 0) return RT_OK;
+  const int64_t want = (int64_t)device_cu_count() * S.cfg.grid_mul;
+  const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(want, S.total));
+  const GroupCfg& c = S.cfg;
+  if (c.bm == 32 && c.bn == 128 && c.wm == 1) launch_tn_t<32, 128, 1>(S, grid, s);
+  else if (c.bm == 64 && c.bn == 128 && c.wm == 2) launch_tn_t<64, 128, 2>(S, grid, s);
+  else if (c.bm == 64 && c.bn == 64 && c.wm == 2) launch_tn_t<64, 64, 2>(S, grid, s);
+  else if (c.bm == 128 && c.bn == 128 && c.wm == 2) launch_tn_t<128, 128, 2>(S, grid, s);
+  else if (c.bm == 128 && c.bn == 64 && c.wm == 2) launch_tn_t<128, 64, 2>(S, grid, s);
+  else if (c.bm == 128 && c.bn == 32 && c.wm == 4) launch_tn_t<128, 32, 4>(S, grid, s);
+  else if (c.bm == 256 && c.bn == 32 && c.wm == 4) launch_tn_t<256, 32, 4>(S, grid, s);
+  else if (c.bm == 256 && c.bn == 64 && c.wm == 4) launch_tn_t<256, 64, 4>(S, grid, s);
+  else return fail(RT_ERR_ARG, "batch: no TN kernel for tile %dx%d (%d waves along M)", c.bm, c.bn,
+                   c.wm);
+  RTA_HIP(hipGetLastError());
+  return RT_OK;
 }
 
 static int upload_stage(Batch* b, int s) {
@@ -379,9 +591,23 @@ static int run_stages(Batch* b, hipStream_t s) {
     hipLaunchKernelGGL(batch_j_kernel, dim3((unsigned)b->n_j), dim3(256), 0, s, b->d_mols,
                        b->d_off_j, b->nmol);
   RTA_HIP(hipGetLastError());
-  RTA_TRY(launch_grouped(b->st[1], s));  // ao2mo stage 1
-  RTA_TRY(launch_grouped(b->st[2], s));  // ao2mo stage 2
-  RTA_TRY(launch_grouped(b->st[3], s));  // K partials
+  if (b->use_tn) {
+    if (b->n_cpad > 0)
+      hipLaunchKernelGGL(batch_cpad_kernel, dim3((unsigned)b->n_cpad), dim3(256), 0, s,
+                         b->d_mols, b->d_off_cpad, b->nmol);
+    RTA_HIP(hipGetLastError());
+    RTA_TRY(launch_tn(b->tn[1], s));  // ao2mo stage 1 -> X^T
+    RTA_TRY(launch_tn(b->tn[2], s));  // ao2mo stage 2 -> Bp
+    if (b->n_bcopy > 0)
+      hipLaunchKernelGGL(batch_bcopy_kernel, dim3((unsigned)b->n_bcopy), dim3(256), 0, s,
+                         b->d_mols, b->d_off_bcopy, b->nmol);
+    RTA_HIP(hipGetLastError());
+    RTA_TRY(launch_tn(b->tn[3], s));  // K partials from Bp
+  } else {
+    RTA_TRY(launch_grouped(b->st[1], s));  // ao2mo stage 1
+    RTA_TRY(launch_grouped(b->st[2], s));  // ao2mo stage 2
+    RTA_TRY(launch_grouped(b->st[3], s));  // K partials
+  }
   if (b->n_kred > 0)
     hipLaunchKernelGGL(batch_kreduce_kernel, dim3((unsigned)b->n_kred), dim3(256), 0, s,
                        b->d_mols, b->d_off_kred, b->nmol);
@@ -391,13 +617,18 @@ static int run_stages(Batch* b, hipStream_t s) {
   return RT_OK;
 }
 
+static int launch_stage(Batch* b, int s, hipStream_t stream) {
+  if (b->use_tn && s >= 1) return launch_tn(b->tn[s], stream);
+  return launch_grouped(b->st[s], stream);
+}
+
 static double time_stage(Batch* b, int s, hipStream_t stream, int reps) {
   hipEvent_t e0, e1;
   if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return 1e30;
-  if (launch_grouped(b->st[s], stream) != RT_OK) return 1e30;  // warm
+  if (launch_stage(b, s, stream) != RT_OK) return 1e30;  // warm
   (void)hipEventRecord(e0, stream);
   for (int r = 0; r < reps; ++r)
-    if (launch_grouped(b->st[s], stream) != RT_OK) return 1e30;
+    if (launch_stage(b, s, stream) != RT_OK) return 1e30;
   (void)hipEventRecord(e1, stream);
   (void)hipEventSynchronize(e1);
   float ms = 0.f;
@@ -420,8 +651,12 @@ extern "C" int rt_batch_create(const rt_mol_desc* mols, int nmol, void** handle)
     delete b;
     return st;
   };
-  int64_t ng = 0, nj = 0, nk = 0;
-  std::vector<int64_t> og(1, 0), oj(1, 0), ok(1, 0);
+  // RTA_BATCH_GROUPED=1 keeps stages 4-6 on the generic grouped engine
+  const char* forced = getenv("RTA_BATCH_GROUPED");
+  const bool use_tn = !(forced && atoi(forced) == 1);
+  b->use_tn = use_tn;
+  int64_t ng = 0, nj = 0, nk = 0, nc = 0, nbc = 0;
+  std::vector<int64_t> og(1, 0), oj(1, 0), ok(1, 0), oc(1, 0), obc(1, 0);
   for (int i = 0; i < nmol; ++i) {
     const rt_mol_desc& m = mols[i];
     if (!(m.nbas > 0 && m.naux > 0 && m.nocc > 0 && m.nocc <= m.nbas && m.ri3ao && m.C &&
@@ -437,28 +672,55 @@ extern "C" int rt_batch_create(const rt_mol_desc* mols, int nmol, void** handle)
     d.nb = m.nbas;
     d.naux = m.naux;
     d.nocc = m.nocc;
-    d.nchunks = std::max<int64_t>(1, std::min<int64_t>(m.naux, 8));
-    int st = dalloc(b, sizeof(double) * m.nbas * m.nocc * m.naux, (void**)&d.X);
+    d.nbp = (m.nbas + 15) / 16 * 16;
+    d.noccp = (m.nocc + 15) / 16 * 16;
+    // K partial slots: up to 8, every slot non-empty (the reduce sums them
+    // all); the TN path slices the depth (P, o) in 16-deep batches, the
+    // generic one the aux slices
+    const int64_t nbt = use_tn ? m.naux * d.noccp / 16 : m.naux;
+    const int64_t want = std::max<int64_t>(1, std::min<int64_t>(nbt, 8));
+    d.bchunk = (nbt + want - 1) / want;
+    d.nchunks = (nbt + d.bchunk - 1) / d.bchunk;
+    const size_t xbytes = sizeof(double) * d.nbp * d.noccp * m.naux;
+    const size_t cbytes = sizeof(double) * d.nbp * m.nbas;
+    int st = dalloc(b, xbytes, (void**)&d.X);
     if (st == RT_OK) st = dalloc(b, sizeof(double) * m.naux, (void**)&d.gamma);
     if (st == RT_OK)
       st = dalloc(b, sizeof(double) * d.nchunks * m.nbas * m.nbas, (void**)&d.part);
+    if (st == RT_OK) st = dalloc(b, sizeof(double) * d.noccp * m.nbas * m.naux, (void**)&d.Bp);
+    if (st == RT_OK) st = dalloc(b, cbytes, (void**)&d.Cp);
+    if (st == RT_OK && use_tn)
+      st = dalloc(b, sizeof(double) * d.nbp * d.nbp * m.naux, (void**)&d.Ap);
+    // the padding of X^T and Cp is zero for the life of the handle (never written)
+    if (st == RT_OK && (hipMemset(d.X, 0, xbytes) != hipSuccess ||
+                        hipMemset(d.Cp, 0, cbytes) != hipSuccess))
+      st = fail(RT_ERR_HIP, "batch: clearing the padded buffers failed");
     if (st != RT_OK) return fail_cleanup(st);
     b->mols.push_back(d);
     const int64_t t32 = (m.nbas + 31) / 32;
     ng += m.naux;
     nj += m.nbas * ((m.nbas + 63) / 64);
     nk += t32 * (t32 + 1) / 2;
+    nc += m.nbas;
+    nbc += m.nbas * ((m.naux + 31) / 32);
+    obc.push_back(nbc);
     og.push_back(ng);
     oj.push_back(nj);
     ok.push_back(nk);
+    oc.push_back(nc);
   }
   b->n_gamma = ng;
   b->n_j = nj;
   b->n_kred = nk;
+  b->n_cpad = nc;
+  b->n_bcopy = nbc;
+
   int st = dalloc(b, sizeof(MolDev) * nmol, (void**)&b->d_mols);
   if (st == RT_OK) st = dalloc(b, sizeof(int64_t) * og.size(), (void**)&b->d_off_gamma);
   if (st == RT_OK) st = dalloc(b, sizeof(int64_t) * oj.size(), (void**)&b->d_off_j);
   if (st == RT_OK) st = dalloc(b, sizeof(int64_t) * ok.size(), (void**)&b->d_off_kred);
+  if (st == RT_OK) st = dalloc(b, sizeof(int64_t) * oc.size(), (void**)&b->d_off_cpad);
+  if (st == RT_OK) st = dalloc(b, sizeof(int64_t) * obc.size(), (void**)&b->d_off_bcopy);
   if (st == RT_OK) st = dalloc(b, sizeof(double) * 2 * nmol, (void**)&b->energies);
   if (st != RT_OK) return fail_cleanup(st);
   if (hipMemcpy(b->d_mols, b->mols.data(), sizeof(MolDev) * nmol, hipMemcpyHostToDevice) !=
@@ -468,12 +730,27 @@ extern "C" int rt_batch_create(const rt_mol_desc* mols, int nmol, void** handle)
       hipMemcpy(b->d_off_j, oj.data(), sizeof(int64_t) * oj.size(), hipMemcpyHostToDevice) !=
           hipSuccess ||
       hipMemcpy(b->d_off_kred, ok.data(), sizeof(int64_t) * ok.size(),
+                hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(b->d_off_cpad, oc.data(), sizeof(int64_t) * oc.size(),
+                hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(b->d_off_bcopy, obc.data(), sizeof(int64_t) * obc.size(),
                 hipMemcpyHostToDevice) != hipSuccess)
     return fail_cleanup(fail(RT_ERR_HIP, "batch: descriptor upload failed"));
   for (int s = 0; s < 4; ++s) {
     build_stage(b, s, GroupCfg{});
     st = upload_stage(b, s);
     if (st != RT_OK) return fail_cleanup(st);
+  }
+  for (int s = 1; s < 4; ++s) {
+    build_tn_stage(b, s, tn_candidates(s)[0]);
+    st = upload_tn_stage(b, s);
+    if (st != RT_OK) return fail_cleanup(st);
+  }
+  if (use_tn) {  // the padded copy of ri3ao
+    hipLaunchKernelGGL(batch_apad_kernel, dim3((unsigned)b->n_gamma), dim3(256), 0, nullptr,
+                       b->d_mols, b->d_off_gamma, b->nmol);
+    if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess)
+      return fail_cleanup(fail(RT_ERR_HIP, "batch: padded ri3ao copy failed"));
   }
   *handle = b;
   return RT_OK;
@@ -489,6 +766,27 @@ extern "C" int rt_batch_autotune(void* h, int reps, void* stream) {
   RTA_TRY(run_stages(b, s));
   RTA_HIP(hipStreamSynchronize(s));
   for (int stg = 0; stg < 4; ++stg) {
+    if (b->use_tn && stg >= 1) {
+      double best = 1e30;
+      GroupCfg bestc;
+      for (const GroupCfg& cand : tn_candidates(stg)) {
+        build_tn_stage(b, stg, cand);
+        RTA_TRY(upload_tn_stage(b, stg));
+        for (int gm : {1, 2, 4}) {
+          b->tn[stg].cfg.grid_mul = gm;
+          const double ms = time_stage(b, stg, s, reps);
+          if (ms < best) {
+            best = ms;
+            bestc = b->tn[stg].cfg;
+          }
+        }
+      }
+      build_tn_stage(b, stg, bestc);
+      RTA_TRY(upload_tn_stage(b, stg));
+      b->tn[stg].cfg = bestc;
+      b->tune_ms[stg] = best;
+      continue;
+    }
     double best = 1e30;
     GroupCfg bestc;
     for (const GroupCfg& cand : stage_candidates(stg)) {
@@ -523,23 +821,40 @@ extern "C" int rt_batch_set_stage(void* h, int stage, int bm, int bn, int waves_
   RTA_CHECK(stage >= 0 && stage < 4, "stage must be 0..3 (density, ao2mo s1, ao2mo s2, K)");
   RTA_CHECK(grid_mul >= 1 && grid_mul <= 64, "grid_mul must be 1..64");
   Batch* b = (Batch*)h;
+  const bool tn = b->use_tn && stage >= 1;
   bool known = false;
-  for (const GroupCfg& c : stage_candidates(stage))
+  for (const GroupCfg& c : tn ? tn_candidates(stage) : stage_candidates(stage))
     known = known || (c.bm == bm && c.bn == bn && c.wm == waves_m);
   RTA_CHECK(known, "stage %d has no %dx%d tile with %d waves along M", stage, bm, bn, waves_m);
   GroupCfg c;
   c.bm = bm;
   c.bn = bn;
   c.wm = waves_m;
-  build_stage(b, stage, c);
-  RTA_TRY(upload_stage(b, stage));
-  b->st[stage].cfg.grid_mul = grid_mul;
+  if (tn) {
+    build_tn_stage(b, stage, c);
+    RTA_TRY(upload_tn_stage(b, stage));
+    b->tn[stage].cfg.grid_mul = grid_mul;
+  } else {
+    build_stage(b, stage, c);
+    RTA_TRY(upload_stage(b, stage));
+    b->st[stage].cfg.grid_mul = grid_mul;
+  }
   if (b->exec) {  // a captured graph holds the old launch shapes
     (void)hipGraphExecDestroy(b->exec);
     (void)hipGraphDestroy(b->graph);
     b->exec = nullptr;
     b->graph = nullptr;
   }
+  return RT_OK;
+}
+
+extern "C" int rt_batch_reload(void* h, void* stream) {
+  RTA_CHECK(h, "NULL handle");
+  Batch* b = (Batch*)h;
+  if (!b->use_tn || b->n_gamma == 0) return RT_OK;
+  hipLaunchKernelGGL(batch_apad_kernel, dim3((unsigned)b->n_gamma), dim3(256), 0,
+                     (hipStream_t)stream, b->d_mols, b->d_off_gamma, b->nmol);
+  RTA_HIP(hipGetLastError());
   return RT_OK;
 }
 
@@ -605,11 +920,13 @@ extern "C" int rt_batch_info(void* h, char* buf, int len) {
   std::string s = "{";
   for (int i = 0; i < 4; ++i) {
     char t[160];
+    const bool tn = b->use_tn && i >= 1;
+    const GroupCfg& c = tn ? b->tn[i].cfg : b->st[i].cfg;
     snprintf(t, sizeof t,
-             "%s\"%s\": {\"tile\": [%d, %d], \"waves_m\": %d, \"grid_mul\": %d, "
-             "\"tuned_ms\": %.4f}",
-             i ? ", " : "", names[i], b->st[i].cfg.bm, b->st[i].cfg.bn, b->st[i].cfg.wm,
-             b->st[i].cfg.grid_mul, b->tune_ms[i]);
+             "%s\"%s\": {\"engine\": \"%s\", \"tile\": [%d, %d], \"waves_m\": %d, "
+             "\"grid_mul\": %d, \"tuned_ms\": %.4f}",
+             i ? ", " : "", names[i], tn ? "tn" : "generic", c.bm, c.bn, c.wm, c.grid_mul,
+             b->tune_ms[i]);
     s += t;
   }
   s += "}";

@@ -89,16 +89,17 @@ __device__ __forceinline__ void tn_issue(const double* __restrict__ base,
   }
 }
 
-template <int BM, int BN, int BK, int NSTAGE>
-__global__ __launch_bounds__(256) void dgemm_tn_kernel(TnParams g) {
+// One work item w (tile x batch chunk) of a TN GEMM; all 256 threads call it.
+// smem: NSTAGE (BM + BN) BK doubles.  WAVES_M of the 4 waves along M.
+template <int BM, int BN, int BK, int NSTAGE, int WAVES_M = 2>
+__device__ __forceinline__ void tn_tile(const TnParams& g, int64_t w, double* __restrict__ smem) {
   static_assert(BK == 16 || BK == 32, "BK");
-  constexpr int WM = BM / 2, WN = BN / 2;
+  constexpr int WAVES_N = 4 / WAVES_M;
+  constexpr int WM = BM / WAVES_M, WN = BN / WAVES_N;
   constexpr int FM = WM / 16, FN = WN / 16;
+  static_assert(WAVES_M * WAVES_N == 4 && WM % 16 == 0 && WN % 16 == 0, "wave layout");
+  static_assert((BM * BK / 2) % 256 == 0 && (BN * BK / 2) % 256 == 0, "operand rows per wave");
   constexpr int STAGE = (BM + BN) * BK;  // doubles per stage
-  extern __shared__ double smem[];
-
-  const int64_t w = xcd_remap(blockIdx.x, gridDim.x);
-  if (w >= g.nwork) return;
   const int64_t chunk = w / g.ntiles;
   const int64_t tid = w % g.ntiles;
   int tm, tn;
@@ -116,7 +117,7 @@ __global__ __launch_bounds__(256) void dgemm_tn_kernel(TnParams g) {
 
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
-  const int wave_m = wave & 1, wave_n = wave >> 1;
+  const int wave_m = wave % WAVES_M, wave_n = wave / WAVES_M;
 
   // Upper mode: a wave tile strictly below the diagonal is never read.  Its
   // wave keeps loading and synchronising but issues no LDS reads or MFMAs,
@@ -154,11 +155,15 @@ __global__ __launch_bounds__(256) void dgemm_tn_kernel(TnParams g) {
   for (int64_t it = 0; it < niter; ++it) {
     // this wave's loads of chunk `it` have landed once at most the loads of
     // the younger chunks are outstanding
-    const int64_t younger = (niter - 1 - it) < (NSTAGE - 2) ? (niter - 1 - it)
-                                                            : (NSTAGE - 2);
-    if (younger >= 2) wait_vmcnt<2 * PER_STAGE>();
-    else if (younger == 1) wait_vmcnt<PER_STAGE>();
-    else wait_vmcnt<0>();
+    if constexpr (NSTAGE > 2) {
+      const int64_t younger = (niter - 1 - it) < (NSTAGE - 2) ? (niter - 1 - it)
+                                                              : (NSTAGE - 2);
+      if (younger >= 2) wait_vmcnt<2 * PER_STAGE>();
+      else if (younger == 1) wait_vmcnt<PER_STAGE>();
+      else wait_vmcnt<0>();
+    } else {
+      wait_vmcnt<0>();
+    }
     // every wave's chunk `it` is in LDS and chunk it-1 is consumed (its LDS
     // reads completed before its MFMAs).  A bare s_barrier: __syncthreads()
     // would also drain vmcnt and serialize the load pipeline.
@@ -222,6 +227,35 @@ __global__ __launch_bounds__(256) void dgemm_tn_kernel(TnParams g) {
         else *p = g.alpha * v + g.beta * (*p);
       }
     }
+  }
+}
+
+template <int BM, int BN, int BK, int NSTAGE>
+__global__ __launch_bounds__(256) void dgemm_tn_kernel(TnParams g) {
+  extern __shared__ double smem[];
+  const int64_t w = xcd_remap(blockIdx.x, gridDim.x);
+  if (w >= g.nwork) return;
+  tn_tile<BM, BN, BK, NSTAGE>(g, w, smem);
+}
+
+// Persistent grouped TN GEMM over a work list: descs[d] has descs[d].nwork
+// items, offs the exclusive prefix sums (offs[ndesc] = total).
+template <int BM, int BN, int BK, int NSTAGE, int WAVES_M>
+__global__ __launch_bounds__(256) void dgemm_grouped_tn_kernel(const TnParams* __restrict__ descs,
+                                                               const int64_t* __restrict__ offs,
+                                                               int ndesc) {
+  extern __shared__ double smem[];
+  const int64_t total = offs[ndesc];
+  for (int64_t gw = blockIdx.x; gw < total; gw += gridDim.x) {
+    int lo = 0, hi = ndesc;
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (offs[mid] <= gw) lo = mid;
+      else hi = mid;
+    }
+    const TnParams g = descs[lo];
+    tn_tile<BM, BN, BK, NSTAGE, WAVES_M>(g, gw - offs[lo], smem);
+    __syncthreads();  // every wave is done with LDS before the next item's loads
   }
 }
 

@@ -105,16 +105,47 @@ def test_batch_eager_autotune_and_graph(D):
     s.destroy()
 
 
-@pytest.mark.parametrize("s1,s2", [((128, 48, 4), (48, 128, 1)), ((256, 32, 4), (32, 256, 1))])
-def test_batch_narrow_tiles(D, s1, s2):
-    """The narrow ao2mo tiles (waves stacked along the wide side), forced,
-    against the single-molecule path; nbas not a multiple of 16 or 64."""
+@pytest.mark.parametrize("s1,s2,k", [((128, 64, 2), (32, 128, 1), (64, 64, 2)),
+                                     ((256, 32, 4), (64, 128, 2), (128, 128, 2)),
+                                     ((128, 32, 4), (128, 128, 2), (64, 64, 2)),
+                                     ((256, 64, 4), (64, 64, 2), (128, 128, 2))])
+def test_batch_tn_tiles(D, s1, s2, k):
+    """Every TN tile of stages 1, 2 and K, forced, against the single-molecule
+    path; nbas and nocc not multiples of 16 (the padded internal layouts)."""
     from rest_tensors_amd.batch import Batch
     mols = make_mols(D, [67, 130, 99, 250], seed0=4200)
     refs = [single(D, m) for m in mols]
     b = Batch(mols)
     b.set_tiles("ao2mo_s1", *s1)
     b.set_tiles("ao2mo_s2", *s2)
+    b.set_tiles("k_partials", *k)
+    info = b.info()
+    assert all(info[k]["engine"] == "tn" for k in ("ao2mo_s1", "ao2mo_s2", "k_partials"))
+    b.run()
+    ej, ex = b.energies()
+    check_against(mols, ej, ex, refs)
+    # new integrals: the padded copy follows after reload
+    for i, m in enumerate(mols):
+        D.gen_ri3ao(m["ri3ao"], m["nbas"], m["naux"], seed=4400 + i)
+    refs2 = [single(D, m) for m in mols]
+    b.reload()
+    b.run()
+    ej, ex = b.energies()
+    check_against(mols, ej, ex, refs2)
+
+
+@pytest.mark.parametrize("s1,s2", [((128, 48, 4), (48, 128, 1)), ((256, 32, 4), (32, 256, 1))])
+def test_batch_generic_stages(D, monkeypatch, s1, s2):
+    """Stages 1, 2 and K forced onto the generic grouped engine (narrow tiles,
+    waves stacked along the wide side) against the single-molecule path."""
+    from rest_tensors_amd.batch import Batch
+    monkeypatch.setenv("RTA_BATCH_GROUPED", "1")
+    mols = make_mols(D, [70, 131, 96], seed0=4300)
+    refs = [single(D, m) for m in mols]
+    b = Batch(mols)
+    b.set_tiles("ao2mo_s1", *s1)
+    b.set_tiles("ao2mo_s2", *s2)
+    assert b.info()["ao2mo_s2"]["engine"] == "generic"
     b.run()
     ej, ex = b.energies()
     check_against(mols, ej, ex, refs)
