@@ -276,7 +276,8 @@ struct TnStage {
 };
 
 static std::vector<GroupCfg> tn_candidates(int s) {
-  if (s == 1) return {{128, 64, 2}, {128, 32, 4}, {256, 32, 4}, {256, 64, 4}};
+  if (s == 1) return {{256, 48, 4}, {128, 48, 4}, {128, 64, 2}, {128, 32, 4}, {256, 32, 4},
+                      {256, 64, 4}};
   if (s == 2) return {{128, 128, 2}, {64, 128, 2}, {64, 64, 2}, {32, 128, 1}};
   return {{64, 64, 2}, {128, 128, 2}};
 }
@@ -523,6 +524,8 @@ static int launch_tn(const TnStage& S, hipStream_t s) {
   else if (c.bm == 128 && c.bn == 32 && c.wm == 4) launch_tn_t<128, 32, 4>(S, grid, s);
   else if (c.bm == 256 && c.bn == 32 && c.wm == 4) launch_tn_t<256, 32, 4>(S, grid, s);
   else if (c.bm == 256 && c.bn == 64 && c.wm == 4) launch_tn_t<256, 64, 4>(S, grid, s);
+  else if (c.bm == 256 && c.bn == 48 && c.wm == 4) launch_tn_t<256, 48, 4>(S, grid, s);
+  else if (c.bm == 128 && c.bn == 48 && c.wm == 4) launch_tn_t<128, 48, 4>(S, grid, s);
   else return fail(RT_ERR_ARG, "batch: no TN kernel for tile %dx%d (%d waves along M)", c.bm, c.bn,
                    c.wm);
   RTA_HIP(hipGetLastError());

@@ -66,17 +66,20 @@ __device__ __forceinline__ int tn_swz(int r) {
 }
 
 // Issues the direct-to-LDS loads of one BK-deep k-chunk of an R-row operand
-// tile: R*BK/2 units of 16 B, R*BK/512 instructions per wave (4 waves).
+// tile: R*BK/2 units of 16 B in NT = R*BK/128 instructions of 64 lanes.  When
+// NT is a multiple of 4 every wave issues NT/4 of them; otherwise (R = 48:
+// NT = 6 for BK = 16) instruction q*4 + wave goes to wave `wave`, and the
+// waves carry different counts, which only the vmcnt(0) wait of a two-stage
+// pipeline allows.
 template <int R, int BK>
 __device__ __forceinline__ void tn_issue(const double* __restrict__ base,
                                          int64_t ld, int64_t r0, int64_t rmax,
                                          int64_t koff, double* lds_tile) {
   constexpr int UPR = BK / 2;                 // units per row
-  constexpr int NI = R * UPR / 256;           // instructions per wave
+  constexpr int NT = R * UPR / 64;            // instructions per workgroup
+  static_assert(NT * 64 == R * UPR, "operand rows");
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-#pragma unroll
-  for (int q = 0; q < NI; ++q) {
-    const int inst = wave * NI + q;
+  auto one = [&](int inst) {
     const int u = inst * 64 + lane;
     const int row = u / UPR, pos = u % UPR;
     const int c = pos ^ tn_swz<BK>(row);
@@ -86,6 +89,14 @@ __device__ __forceinline__ void tn_issue(const double* __restrict__ base,
     __builtin_amdgcn_global_load_lds((const glb_void_t*)src,
                                      (lds_void_t*)(lds_tile + inst * 128), 16,
                                      0, 0);
+  };
+  if constexpr (NT % 4 == 0) {
+#pragma unroll
+    for (int q = 0; q < NT / 4; ++q) one(wave * (NT / 4) + q);
+  } else {
+#pragma unroll
+    for (int q = 0; q < (NT + 3) / 4; ++q)
+      if (q * 4 + wave < NT) one(q * 4 + wave);
   }
 }
 
@@ -98,7 +109,8 @@ __device__ __forceinline__ void tn_tile(const TnParams& g, int64_t w, double* __
   constexpr int WM = BM / WAVES_M, WN = BN / WAVES_N;
   constexpr int FM = WM / 16, FN = WN / 16;
   static_assert(WAVES_M * WAVES_N == 4 && WM % 16 == 0 && WN % 16 == 0, "wave layout");
-  static_assert((BM * BK / 2) % 256 == 0 && (BN * BK / 2) % 256 == 0, "operand rows per wave");
+  static_assert(((BM * BK / 2) % 256 == 0 && (BN * BK / 2) % 256 == 0) || NSTAGE == 2,
+                "uneven load counts per wave need the two-stage (vmcnt 0) pipeline");
   constexpr int STAGE = (BM + BN) * BK;  // doubles per stage
   const int64_t chunk = w / g.ntiles;
   const int64_t tid = w % g.ntiles;

@@ -108,7 +108,9 @@ def test_batch_eager_autotune_and_graph(D):
 @pytest.mark.parametrize("s1,s2,k", [((128, 64, 2), (32, 128, 1), (64, 64, 2)),
                                      ((256, 32, 4), (64, 128, 2), (128, 128, 2)),
                                      ((128, 32, 4), (128, 128, 2), (64, 64, 2)),
-                                     ((256, 64, 4), (64, 64, 2), (128, 128, 2))])
+                                     ((256, 64, 4), (64, 64, 2), (128, 128, 2)),
+                                     ((256, 48, 4), (128, 128, 2), (64, 64, 2)),
+                                     ((128, 48, 4), (64, 128, 2), (64, 64, 2))])
 def test_batch_tn_tiles(D, s1, s2, k):
     """Every TN tile of stages 1, 2 and K, forced, against the single-molecule
     path; nbas and nocc not multiples of 16 (the padded internal layouts)."""
