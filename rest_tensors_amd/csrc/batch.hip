@@ -76,29 +76,46 @@ __device__ __forceinline__ double b_wave_sum(double v) {
   return v;
 }
 
-// one block per (molecule, aux slice)
+// one block per (molecule, pair of aux slices): each D element read serves two
+// slices; upper triangle of A_P with weights 2 (off-diagonal) and 1
 __global__ __launch_bounds__(256) void batch_gamma_kernel(
     const MolDev* __restrict__ mols, const int64_t* __restrict__ offs, int nmol) {
-  __shared__ double red[4];
+  __shared__ double red[2][4];
   const int m = find_mol(offs, nmol, blockIdx.x);
   const MolDev M = mols[m];
-  const int64_t P = blockIdx.x - offs[m];
-  const int64_t nb = M.nb;
-  const double* Ap = M.A + P * nb * nb;
+  const int64_t P = 2 * (blockIdx.x - offs[m]);
+  const int64_t nb = M.nb, n2 = nb * nb;
+  const bool two = P + 1 < M.naux;
+  const double* Ap = M.A + P * n2;
+  const double* Aq = two ? Ap + n2 : Ap;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  double acc = 0.0;
+  double acc = 0.0, acc2 = 0.0;
   for (int64_t j = wave; j < nb; j += 4) {
-    const double* col = Ap + j * nb;
+    const double* cp = Ap + j * nb;
+    const double* cq = Aq + j * nb;
     const double* dcol = M.D + j * nb;
-    for (int64_t i = lane; i <= j; i += 64) acc += (i == j ? 1.0 : 2.0) * col[i] * dcol[i];
+    for (int64_t i = lane; i <= j; i += 64) {
+      const double dw = (i == j) ? dcol[i] : 2.0 * dcol[i];
+      acc += cp[i] * dw;
+      acc2 += cq[i] * dw;
+    }
   }
   acc = b_wave_sum(acc);
-  if (lane == 0) red[wave] = acc;
+  acc2 = b_wave_sum(acc2);
+  if (lane == 0) {
+    red[0][wave] = acc;
+    red[1][wave] = acc2;
+  }
   __syncthreads();
-  if (threadIdx.x == 0) M.gamma[P] = (red[0] + red[1]) + (red[2] + red[3]);
+  if (threadIdx.x == 0) M.gamma[P] = (red[0][0] + red[0][1]) + (red[0][2] + red[0][3]);
+  if (threadIdx.x == 1 && two)
+    M.gamma[P + 1] = (red[1][0] + red[1][1]) + (red[1][2] + red[1][3]);
 }
 
-// one block per (molecule, column j, 64-row segment)
+// one block per (molecule, column j, 64-row segment): 4 waves take every 4th
+// aux slice with 8 loads in flight per lane (the wave index is made uniform
+// so gamma comes through scalar loads), combined through LDS; writes J[i,j]
+// and J[j,i]
 __global__ __launch_bounds__(256) void batch_j_kernel(const MolDev* __restrict__ mols,
                                                       const int64_t* __restrict__ offs,
                                                       int nmol) {
@@ -109,14 +126,27 @@ __global__ __launch_bounds__(256) void batch_j_kernel(const MolDev* __restrict__
   const int64_t loc = blockIdx.x - offs[m];
   const int64_t j = loc / nseg, i0 = (loc % nseg) * 64;
   if (i0 > j) return;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int64_t i = i0 + lane, n2 = nb * nb;
-  double acc = 0.0;
+  const double* gm = M.gamma;
+  double acc = 0.0, acc2 = 0.0;
   if (i <= j) {
     const double* p = M.A + i + j * nb;
-    for (int64_t P = wave; P < M.naux; P += 4) acc += p[P * n2] * M.gamma[P];
+    int64_t P = wave;
+    for (; P + 28 < M.naux; P += 32) {
+      double a[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) a[u] = p[(P + 4 * u) * n2];
+#pragma unroll
+      for (int u = 0; u < 8; u += 2) {
+        acc += a[u] * gm[P + 4 * u];
+        acc2 += a[u + 1] * gm[P + 4 * u + 4];
+      }
+    }
+    for (; P < M.naux; P += 4) acc += p[P * n2] * gm[P];
   }
-  red[wave][lane] = acc;
+  red[wave][lane] = acc + acc2;
   __syncthreads();
   if (wave == 0 && i <= j) {
     const double v = (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
@@ -297,6 +327,8 @@ struct Batch {
   int64_t n_cpad = 0;
   int64_t* d_off_bcopy = nullptr;  // (molecule, column, 32 slices) offsets for the B copy
   int64_t n_bcopy = 0;
+  int64_t* d_off_gpair = nullptr;  // (molecule, pair of aux slices) offsets for gamma
+  int64_t n_gpair = 0;
   double tune_ms[4] = {0, 0, 0, 0};
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -587,9 +619,9 @@ static int launch_grouped(const GroupedStage& S, hipStream_t s) {
 
 static int run_stages(Batch* b, hipStream_t s) {
   RTA_TRY(launch_grouped(b->st[0], s));  // density
-  if (b->n_gamma > 0)
-    hipLaunchKernelGGL(batch_gamma_kernel, dim3((unsigned)b->n_gamma), dim3(256), 0, s,
-                       b->d_mols, b->d_off_gamma, b->nmol);
+  if (b->n_gpair > 0)
+    hipLaunchKernelGGL(batch_gamma_kernel, dim3((unsigned)b->n_gpair), dim3(256), 0, s,
+                       b->d_mols, b->d_off_gpair, b->nmol);
   if (b->n_j > 0)
     hipLaunchKernelGGL(batch_j_kernel, dim3((unsigned)b->n_j), dim3(256), 0, s, b->d_mols,
                        b->d_off_j, b->nmol);
@@ -658,8 +690,8 @@ extern "C" int rt_batch_create(const rt_mol_desc* mols, int nmol, void** handle)
   const char* forced = getenv("RTA_BATCH_GROUPED");
   const bool use_tn = !(forced && atoi(forced) == 1);
   b->use_tn = use_tn;
-  int64_t ng = 0, nj = 0, nk = 0, nc = 0, nbc = 0;
-  std::vector<int64_t> og(1, 0), oj(1, 0), ok(1, 0), oc(1, 0), obc(1, 0);
+  int64_t ng = 0, nj = 0, nk = 0, nc = 0, nbc = 0, ngp = 0;
+  std::vector<int64_t> og(1, 0), oj(1, 0), ok(1, 0), oc(1, 0), obc(1, 0), ogp(1, 0);
   for (int i = 0; i < nmol; ++i) {
     const rt_mol_desc& m = mols[i];
     if (!(m.nbas > 0 && m.naux > 0 && m.nocc > 0 && m.nocc <= m.nbas && m.ri3ao && m.C &&
@@ -707,6 +739,8 @@ extern "C" int rt_batch_create(const rt_mol_desc* mols, int nmol, void** handle)
     nc += m.nbas;
     nbc += m.nbas * ((m.naux + 31) / 32);
     obc.push_back(nbc);
+    ngp += (m.naux + 1) / 2;
+    ogp.push_back(ngp);
     og.push_back(ng);
     oj.push_back(nj);
     ok.push_back(nk);
@@ -717,6 +751,7 @@ extern "C" int rt_batch_create(const rt_mol_desc* mols, int nmol, void** handle)
   b->n_kred = nk;
   b->n_cpad = nc;
   b->n_bcopy = nbc;
+  b->n_gpair = ngp;
 
   int st = dalloc(b, sizeof(MolDev) * nmol, (void**)&b->d_mols);
   if (st == RT_OK) st = dalloc(b, sizeof(int64_t) * og.size(), (void**)&b->d_off_gamma);
@@ -724,6 +759,7 @@ extern "C" int rt_batch_create(const rt_mol_desc* mols, int nmol, void** handle)
   if (st == RT_OK) st = dalloc(b, sizeof(int64_t) * ok.size(), (void**)&b->d_off_kred);
   if (st == RT_OK) st = dalloc(b, sizeof(int64_t) * oc.size(), (void**)&b->d_off_cpad);
   if (st == RT_OK) st = dalloc(b, sizeof(int64_t) * obc.size(), (void**)&b->d_off_bcopy);
+  if (st == RT_OK) st = dalloc(b, sizeof(int64_t) * ogp.size(), (void**)&b->d_off_gpair);
   if (st == RT_OK) st = dalloc(b, sizeof(double) * 2 * nmol, (void**)&b->energies);
   if (st != RT_OK) return fail_cleanup(st);
   if (hipMemcpy(b->d_mols, b->mols.data(), sizeof(MolDev) * nmol, hipMemcpyHostToDevice) !=
@@ -737,6 +773,8 @@ extern "C" int rt_batch_create(const rt_mol_desc* mols, int nmol, void** handle)
       hipMemcpy(b->d_off_cpad, oc.data(), sizeof(int64_t) * oc.size(),
                 hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(b->d_off_bcopy, obc.data(), sizeof(int64_t) * obc.size(),
+                hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(b->d_off_gpair, ogp.data(), sizeof(int64_t) * ogp.size(),
                 hipMemcpyHostToDevice) != hipSuccess)
     return fail_cleanup(fail(RT_ERR_HIP, "batch: descriptor upload failed"));
   for (int s = 0; s < 4; ++s) {
