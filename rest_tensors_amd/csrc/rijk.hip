@@ -63,6 +63,78 @@ __global__ __launch_bounds__(256) void rij_gamma_kernel(
     gamma[P + 1] = (red[1][0] + red[1][1]) + (red[1][2] + red[1][3]);
 }
 
+// The same sum for an even nbas and 16-byte aligned A and D: one block per SP
+// aux slices, each D element read once for all SP.  A lane takes the pair of
+// rows (2t, 2t+1) of a column with one 16-byte load per slice (weights 2 above
+// the diagonal, 1 on it, 0 for row j+1 of an even column j, which the even
+// nbas keeps inside the column); wave w walks columns j = w, w+4, ... in
+// chunks of 64 pairs, the loads of the next chunk issued before the current
+// one is summed.  Fixed order -> deterministic.
+template <int SP>
+__global__ __launch_bounds__(256) void rij_gamma_x2_kernel(
+    const double* __restrict__ A, const double* __restrict__ D, int64_t nbas,
+    int64_t naux, double* __restrict__ gamma) {
+  __shared__ double red[SP][4];
+  const int64_t P0 = (int64_t)blockIdx.x * SP;
+  const int64_t n2 = nbas * nbas, ldh = nbas / 2;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const double2* Ab[SP];
+#pragma unroll
+  for (int q = 0; q < SP; ++q)
+    Ab[q] = reinterpret_cast<const double2*>(A + (P0 + q < naux ? P0 + q : P0) * n2);
+  const double2* D2 = reinterpret_cast<const double2*>(D);
+  double acc[SP];
+#pragma unroll
+  for (int q = 0; q < SP; ++q) acc[q] = 0.0;
+  auto load = [&](int64_t j, int64_t u, double2* a, double2& d) {
+    if (2 * (u + lane) <= j) {
+      const int64_t off = j * ldh + u + lane;
+      d = D2[off];
+#pragma unroll
+      for (int q = 0; q < SP; ++q) a[q] = Ab[q][off];
+    } else {
+      d = make_double2(0.0, 0.0);
+#pragma unroll
+      for (int q = 0; q < SP; ++q) a[q] = make_double2(0.0, 0.0);
+    }
+  };
+  int64_t j = wave, u = 0;
+  double2 a[SP], d;
+  if (j < nbas) load(j, u, a, d);
+  while (j < nbas) {
+    int64_t jn = j, un = u + 64;
+    if (2 * un > jn) {
+      un = 0;
+      jn += 4;
+    }
+    double2 an[SP], dn = make_double2(0.0, 0.0);
+#pragma unroll
+    for (int q = 0; q < SP; ++q) an[q] = dn;
+    if (jn < nbas) load(jn, un, an, dn);
+    const int64_t r = 2 * (u + lane);
+    const double d0 = (r < j ? 2.0 : (r == j ? 1.0 : 0.0)) * d.x;
+    const double d1 = (r + 1 < j ? 2.0 : (r + 1 == j ? 1.0 : 0.0)) * d.y;
+#pragma unroll
+    for (int q = 0; q < SP; ++q) acc[q] += a[q].x * d0 + a[q].y * d1;
+    j = jn;
+    u = un;
+    d = dn;
+#pragma unroll
+    for (int q = 0; q < SP; ++q) a[q] = an[q];
+  }
+#pragma unroll
+  for (int q = 0; q < SP; ++q) acc[q] = wave_sum(acc[q]);
+  if (lane == 0) {
+#pragma unroll
+    for (int q = 0; q < SP; ++q) red[q][wave] = acc[q];
+  }
+  __syncthreads();
+  if (threadIdx.x < SP && P0 + threadIdx.x < naux) {
+    const int q = threadIdx.x;
+    gamma[P0 + q] = (red[q][0] + red[q][1]) + (red[q][2] + red[q][3]);
+  }
+}
+
 // Block = (column j, 64-row segment s of rows 0..j); its 4 waves take every
 // 4th aux slice (8 loads in flight per lane; the wave-uniform gamma index
 // lets gamma come through scalar loads) and are combined through LDS.
@@ -180,8 +252,13 @@ extern "C" int rt_rij_dev(const double* ri3ao, int64_t nbas, int64_t naux,
     RTA_TRY(scratch_get(SCRATCH_JK, sizeof(double) * naux, (void**)&g));
   if (naux > 0) {
     KernelTimer kt("rij_gamma", s);
-    hipLaunchKernelGGL(rij_gamma_kernel, dim3((unsigned)((naux + 1) / 2)), dim3(256), 0, s,
-                       ri3ao, D, nbas, naux, g);
+    const bool x2 = nbas % 2 == 0 && ((uintptr_t)ri3ao & 15) == 0 && ((uintptr_t)D & 15) == 0;
+    if (x2)
+      hipLaunchKernelGGL(rij_gamma_x2_kernel<8>, dim3((unsigned)((naux + 7) / 8)), dim3(256), 0,
+                         s, ri3ao, D, nbas, naux, g);
+    else
+      hipLaunchKernelGGL(rij_gamma_kernel, dim3((unsigned)((naux + 1) / 2)), dim3(256), 0, s,
+                         ri3ao, D, nbas, naux, g);
     RTA_HIP(hipGetLastError());
   }
   {

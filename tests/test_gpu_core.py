@@ -344,6 +344,18 @@ def test_odd_shapes_pipeline(D):
     check_pipeline(g, 5, square_subset=13)
 
 
+@pytest.mark.parametrize("nbas,naux", [(2, 1), (2, 9), (6, 3), (64, 7), (130, 17), (258, 8)])
+def test_rij_even_nbas_shapes(D, nbas, naux):
+    """Even nbas takes the 16-byte gamma path (8 slices per block): naux not a
+    multiple of 8, columns shorter than one 64-pair chunk and longer than
+    two, against the oracle."""
+    g = pipeline(D, nbas, naux, 1, seed=700 + nbas + naux)
+    gamma, J, e_j = O.rij(g["A"], g["D"])
+    assert relf(g["gamma"], gamma) < TOL
+    assert relf(g["J"], J) < TOL
+    assert abs(g["e_j"] - e_j) <= TOL * abs(e_j)
+
+
 # ---- packed upper -------------------------------------------------------------------
 @pytest.mark.parametrize("n", [1, 5, 31, 32, 33, 100, 1024])
 def test_upper_unpack_pack_bit_exact(D, n):
