@@ -208,6 +208,9 @@ def main():
         log(f"[bench] ao2mo {ao_ms:.2f} ms, RI-J {j_ms:.2f} ms "
             f"({extras['rij_gbs']} GB/s)")
 
+    if not args.no_extras:
+        extras["hbm_kernels"] = bench_hbm_kernels(D, dA, nbas, naux, args)
+
     del dA
     if not args.no_extras and args.batch_mols > 0:
         extras["batched"] = bench_batched(D, O, args)
@@ -267,6 +270,58 @@ def main():
         "extras": extras,
     }
     print(json.dumps(out), flush=True)
+
+
+def _timed(D, fn, reps=3):
+    """Average event-timed ms of fn() on the default stream (after one warm call)."""
+    fn()
+    D.synchronize()
+    e0, e1 = D.Event(), D.Event()
+    e0.record()
+    for _ in range(reps):
+        fn()
+    e1.record()
+    return e0.elapsed_ms(e1) / reps
+
+
+def bench_hbm_kernels(D, dA, nbas, naux, args):
+    """The bandwidth-bound kernels on ri3ao [nbas, nbas, naux] (resident): the
+    packed-upper copy [nbas(nbas+1)/2, naux] (rifull_to_matfull_symm), the two
+    GEMVs over it that compose J (gamma = A^T d, J = A gamma), the rank-3
+    transposes (16 B per element) and the packed-upper unpack of one matrix.
+    Bytes are algorithmic (each element read once, written once)."""
+    npair = nbas * (nbas + 1) // 2
+    out = {}
+
+    def rec(name, ms, nbytes):
+        out[name] = {"ms": round(ms, 4), "GB/s": round(nbytes / (ms * 1e-3) / 1e9, 1),
+                     "frac_of_8TBs": round(nbytes / (ms * 1e-3) / (PEAK_HBM_GBS * 1e9), 3)}
+
+    dP = D.DeviceArray(npair * naux)
+    ms = _timed(D, lambda: D.rifull_to_matfull_symm(dA, nbas, naux, dP))
+    rec("rifull_to_matfull_symm", ms, 16.0 * npair * naux)
+    dx = D.DeviceArray(npair)
+    D.gen_normal(dx, npair, seed=11)
+    dg = D.DeviceArray(naux)
+    ms = _timed(D, lambda: D.dgemv("T", npair, naux, 1.0, dP, npair, dx, 0.0, dg))
+    rec("dgemv_T_packed", ms, 8.0 * npair * naux)
+    ms = _timed(D, lambda: D.dgemv("N", npair, naux, 1.0, dP, npair, dg, 0.0, dx))
+    rec("dgemv_N_packed", ms, 8.0 * npair * naux)
+    del dP, dx, dg
+    dT = D.DeviceArray(nbas * nbas * naux)
+    for order in ("jik", "kji"):
+        ms = _timed(D, lambda: D.rifull_transpose(dA, nbas, nbas, naux, order, dT), reps=2)
+        rec(f"rifull_transpose_{order}", ms, 16.0 * nbas * nbas * naux)
+    del dT
+    dU = D.DeviceArray(nbas * nbas)
+    dUp = D.DeviceArray(npair)
+    D.gen_normal(dUp, npair, seed=12)
+    ms = _timed(D, lambda: D.upper_unpack(dUp, npair, dU), reps=20)
+    rec("upper_unpack", ms, 8.0 * npair + 8.0 * nbas * nbas)
+    del dU, dUp
+    D.synchronize()
+    log("[bench] hbm kernels: " + ", ".join(f"{k} {v['GB/s']} GB/s" for k, v in out.items()))
+    return out
 
 
 def bench_batched(D, O, args):
@@ -365,6 +420,23 @@ def bench_davidson(D, O, args):
                 "converged": all(res.conv), "max_residual": res.stats["max_residual"]}
 
     target = problem(0x5EED0002)
+    # the operator and the shift-invert correction on blocks of k vectors:
+    # each pass over B (n x naux, FP64) reads it once
+    n_t = target["n"]
+    kern = {}
+    si_k, _ = shift_invert(target)
+    for k in (1, 8):
+        dX = D.DeviceArray(n_t * k)
+        D.gen_normal(dX, n_t * k, seed=21)
+        dY = D.DeviceArray(n_t * k)
+        ms = _timed(D, lambda: target["op"].apply(dX, dY, k), reps=10)
+        kern[f"casida_matvec_k{k}"] = {"ms": round(ms, 4), "GB/s": round(
+            2 * 8.0 * n_t * naux / (ms * 1e-3) / 1e9, 1)}
+        ms = _timed(D, lambda: si_k.apply(dX, dY, k), reps=10)
+        kern[f"shift_invert_apply_k{k}"] = {"ms": round(ms, 4), "GB/s": round(
+            2 * 8.0 * n_t * naux / (ms * 1e-3) / 1e9, 1)}
+        del dX, dY
+    del si_k
     solve(target)  # warm (kernels, scratch)
     plain = solve(target)
     si_t, t_si = shift_invert(target)
@@ -405,6 +477,7 @@ def bench_davidson(D, O, args):
            "shift_invert_over_plain_time": round((sionly.stats["seconds"] + t_si) /
                                                  plain.stats["seconds"], 3),
            "training_s": round(t_train, 2), "training_samples": nsamp,
+           "kernels": kern,
            "max_eig_diff": float(max(np.abs(plain.e - learned.e).max(),
                                      np.abs(plain.e - sionly.e).max()))}
     log(f"[bench] davidson n={target['n']}: plain {out['plain']['seconds']} s "

@@ -3,8 +3,8 @@
 //
 // Every stage of the FP64 pipeline is ONE launch covering all molecules:
 //   1 density      D_m = C_occ C_occ^T          grouped GEMM (persistent)
-//   2 gamma        gamma_m,P = <A_m,P, D_m>      grouped, one block per (m, P)
-//   3 J            J_m = sum_P A_m,P gamma_m,P   grouped, one block per (m, j, seg)
+//   2 gamma        gamma_m,P = <A_m,P, D_m>      grouped, one block per (m, 4 slices)
+//   3 J            J_m = sum_P A_m,P gamma_m,P   grouped, one lane per packed (i, j)
 //   4 ao2mo s1     X^T = A_P^T C_occ             grouped TN GEMM (direct-to-LDS)
 //   5 ao2mo s2     B_P = (X^T)^T C               grouped TN GEMM
 //   6 K partials   sum_P B_P^T B_P (upper tiles) grouped TN GEMM, batch-reducing
@@ -112,47 +112,119 @@ __global__ __launch_bounds__(256) void batch_gamma_kernel(
     M.gamma[P + 1] = (red[1][0] + red[1][1]) + (red[1][2] + red[1][3]);
 }
 
-// one block per (molecule, column j, 64-row segment): 4 waves take every 4th
-// aux slice with 8 loads in flight per lane (the wave index is made uniform
-// so gamma comes through scalar loads), combined through LDS; writes J[i,j]
-// and J[j,i]
+// The same sums from the padded copy Ap (leading dimension nbp, a multiple of
+// 16, so every column starts 16-byte aligned), one block per (molecule, SP
+// aux slices): a lane takes the pair of rows (2t, 2t+1) of a column with one
+// 16-byte load per slice (weights 2 above the diagonal, 1 on it, 0 for the
+// row below it, which is the zero pad or the next row and never past the
+// column); wave w walks columns j = w, w+4, ... in chunks of 64 pairs, the
+// next chunk's loads issued before the current one is summed.  D is the
+// unpadded [nb, nb] (its rows read one by one, once for all SP slices).
+template <int SP>
+__global__ __launch_bounds__(256) void batch_gamma_x2_kernel(
+    const MolDev* __restrict__ mols, const int64_t* __restrict__ offs, int nmol) {
+  __shared__ double red[SP][4];
+  const int m = find_mol(offs, nmol, blockIdx.x);
+  const MolDev M = mols[m];
+  const int64_t P0 = SP * (blockIdx.x - offs[m]);
+  const int64_t nb = M.nb, ldh = M.nbp / 2, naux = M.naux;
+  const double2* Ab[SP];
+#pragma unroll
+  for (int q = 0; q < SP; ++q)
+    Ab[q] = reinterpret_cast<const double2*>(M.Ap +
+                                             (P0 + q < naux ? P0 + q : P0) * M.nbp * M.nbp);
+  const double* D = M.D;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  auto load = [&](int64_t j, int64_t u, double2* a, double& d0, double& d1) {
+    const int64_t r = 2 * (u + lane);
+    if (r <= j) {
+#pragma unroll
+      for (int q = 0; q < SP; ++q) a[q] = Ab[q][j * ldh + u + lane];
+      const double* dc = D + j * nb + r;
+      d0 = (r < j ? 2.0 : 1.0) * dc[0];
+      d1 = r + 1 < j ? 2.0 * dc[1] : (r + 1 == j ? dc[1] : 0.0);
+    }
+  };
+  double acc[SP];
+  double2 a[SP];
+#pragma unroll
+  for (int q = 0; q < SP; ++q) {
+    acc[q] = 0.0;
+    a[q] = make_double2(0.0, 0.0);
+  }
+  double d0 = 0.0, d1 = 0.0;
+  int64_t j = wave, u = 0;
+  if (j < nb) load(j, u, a, d0, d1);
+  while (j < nb) {
+    int64_t jn = j, un = u + 64;
+    if (2 * un > jn) {
+      un = 0;
+      jn += 4;
+    }
+    double2 an[SP];
+#pragma unroll
+    for (int q = 0; q < SP; ++q) an[q] = make_double2(0.0, 0.0);
+    double e0 = 0.0, e1 = 0.0;
+    if (jn < nb) load(jn, un, an, e0, e1);
+#pragma unroll
+    for (int q = 0; q < SP; ++q) {
+      acc[q] += a[q].x * d0 + a[q].y * d1;
+      a[q] = an[q];
+    }
+    j = jn;
+    u = un;
+    d0 = e0;
+    d1 = e1;
+  }
+#pragma unroll
+  for (int q = 0; q < SP; ++q) acc[q] = b_wave_sum(acc[q]);
+  if (lane == 0) {
+#pragma unroll
+    for (int q = 0; q < SP; ++q) red[q][wave] = acc[q];
+  }
+  __syncthreads();
+  if (threadIdx.x < SP && P0 + threadIdx.x < naux) {
+    const int q = threadIdx.x;
+    M.gamma[P0 + q] = (red[q][0] + red[q][1]) + (red[q][2] + red[q][3]);
+  }
+}
+constexpr int kGammaSlices = 4;  // aux slices per block of batch_gamma_x2_kernel
+
+// one block per (molecule, 256 consecutive pairs of the packed upper
+// triangle, q = j(j+1)/2 + i): every lane owns one pair (i, j) and sums all
+// aux slices with 16 loads in flight, so short columns leave no lane idle;
+// writes J[i,j] and J[j,i]
 __global__ __launch_bounds__(256) void batch_j_kernel(const MolDev* __restrict__ mols,
                                                       const int64_t* __restrict__ offs,
                                                       int nmol) {
-  __shared__ double red[4][64];
+  constexpr int U = 16;
   const int m = find_mol(offs, nmol, blockIdx.x);
   const MolDev M = mols[m];
-  const int64_t nb = M.nb, nseg = (nb + 63) / 64;
-  const int64_t loc = blockIdx.x - offs[m];
-  const int64_t j = loc / nseg, i0 = (loc % nseg) * 64;
-  if (i0 > j) return;
-  const int lane = threadIdx.x & 63;
-  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int64_t i = i0 + lane, n2 = nb * nb;
+  const int64_t nb = M.nb, n2 = nb * nb, naux = M.naux;
+  const int64_t q = (blockIdx.x - offs[m]) * 256 + threadIdx.x;
+  if (q >= nb * (nb + 1) / 2) return;
+  int64_t j = (int64_t)((sqrt(8.0 * (double)q + 1.0) - 1.0) * 0.5);
+  while (j * (j + 1) / 2 > q) --j;
+  while ((j + 1) * (j + 2) / 2 <= q) ++j;
+  const int64_t i = q - j * (j + 1) / 2;
+  const double* p = M.A + i + j * nb;
   const double* gm = M.gamma;
   double acc = 0.0, acc2 = 0.0;
-  if (i <= j) {
-    const double* p = M.A + i + j * nb;
-    int64_t P = wave;
-    for (; P + 28 < M.naux; P += 32) {
-      double a[8];
+  int64_t P = 0;
+  for (; P + U - 1 < naux; P += U) {
+    double a[U];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) a[u] = p[(P + 4 * u) * n2];
+    for (int u = 0; u < U; ++u) a[u] = p[(P + u) * n2];
 #pragma unroll
-      for (int u = 0; u < 8; u += 2) {
-        acc += a[u] * gm[P + 4 * u];
-        acc2 += a[u + 1] * gm[P + 4 * u + 4];
-      }
+    for (int u = 0; u < U; u += 2) {
+      acc += a[u] * gm[P + u];
+      acc2 += a[u + 1] * gm[P + u + 1];
     }
-    for (; P < M.naux; P += 4) acc += p[P * n2] * gm[P];
   }
-  red[wave][lane] = acc + acc2;
-  __syncthreads();
-  if (wave == 0 && i <= j) {
-    const double v = (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
-    M.J[i + j * nb] = v;
-    if (i != j) M.J[j + i * nb] = v;
-  }
+  for (; P < naux; ++P) acc += p[P * n2] * gm[P];
+  const double v = acc + acc2;
+  M.J[i + j * nb] = v;
+  if (i != j) M.J[j + i * nb] = v;
 }
 
 // one block per (molecule, upper 32x32 tile): K = sum of partial slots, mirrored
@@ -329,6 +401,8 @@ struct Batch {
   int64_t n_bcopy = 0;
   int64_t* d_off_gpair = nullptr;  // (molecule, pair of aux slices) offsets for gamma
   int64_t n_gpair = 0;
+  int64_t* d_off_gquad = nullptr;  // (molecule, kGammaSlices aux slices), padded gamma
+  int64_t n_gquad = 0;
   double tune_ms[4] = {0, 0, 0, 0};
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -530,34 +604,39 @@ static int upload_tn_stage(Batch* b, int s) {
   return RT_OK;
 }
 
-template <int BM, int BN, int WM>
+template <int BM, int BN, int WM, bool CS>
 static void launch_tn_t(const TnStage& S, unsigned grid, hipStream_t s) {
   constexpr size_t lds = 2 * (BM + BN) * 16 * sizeof(double);
   static bool attr = false;
   if (!attr) {
-    (void)hipFuncSetAttribute((const void*)dgemm_grouped_tn_kernel<BM, BN, 16, 2, WM>,
+    (void)hipFuncSetAttribute((const void*)dgemm_grouped_tn_kernel<BM, BN, 16, 2, WM, CS>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     attr = true;
   }
-  hipLaunchKernelGGL((dgemm_grouped_tn_kernel<BM, BN, 16, 2, WM>), dim3(grid), dim3(256), lds, s,
-                     S.d_descs, S.d_offs, (int)S.descs.size());
+  hipLaunchKernelGGL((dgemm_grouped_tn_kernel<BM, BN, 16, 2, WM, CS>), dim3(grid), dim3(256),
+                     lds, s, S.d_descs, S.d_offs, (int)S.descs.size());
 }
 
-static int launch_tn(const TnStage& S, hipStream_t s) {
+// Stage 1 (N = nocc of each molecule, below the tile width for most) runs the
+// column-skipping form; stage 2 and the K slots fill their tiles.
+static int launch_tn(const TnStage& S, hipStream_t s, bool colskip) {
   if (S.total == 0) return RT_OK;
   const int64_t want = (int64_t)device_cu_count() * S.cfg.grid_mul;
   const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(want, S.total));
   const GroupCfg& c = S.cfg;
-  if (c.bm == 32 && c.bn == 128 && c.wm == 1) launch_tn_t<32, 128, 1>(S, grid, s);
-  else if (c.bm == 64 && c.bn == 128 && c.wm == 2) launch_tn_t<64, 128, 2>(S, grid, s);
-  else if (c.bm == 64 && c.bn == 64 && c.wm == 2) launch_tn_t<64, 64, 2>(S, grid, s);
-  else if (c.bm == 128 && c.bn == 128 && c.wm == 2) launch_tn_t<128, 128, 2>(S, grid, s);
-  else if (c.bm == 128 && c.bn == 64 && c.wm == 2) launch_tn_t<128, 64, 2>(S, grid, s);
-  else if (c.bm == 128 && c.bn == 32 && c.wm == 4) launch_tn_t<128, 32, 4>(S, grid, s);
-  else if (c.bm == 256 && c.bn == 32 && c.wm == 4) launch_tn_t<256, 32, 4>(S, grid, s);
-  else if (c.bm == 256 && c.bn == 64 && c.wm == 4) launch_tn_t<256, 64, 4>(S, grid, s);
-  else if (c.bm == 256 && c.bn == 48 && c.wm == 4) launch_tn_t<256, 48, 4>(S, grid, s);
-  else if (c.bm == 128 && c.bn == 48 && c.wm == 4) launch_tn_t<128, 48, 4>(S, grid, s);
+  if (colskip) {
+    if (c.bm == 256 && c.bn == 48 && c.wm == 4) launch_tn_t<256, 48, 4, true>(S, grid, s);
+    else if (c.bm == 128 && c.bn == 48 && c.wm == 4) launch_tn_t<128, 48, 4, true>(S, grid, s);
+    else if (c.bm == 128 && c.bn == 64 && c.wm == 2) launch_tn_t<128, 64, 2, true>(S, grid, s);
+    else if (c.bm == 128 && c.bn == 32 && c.wm == 4) launch_tn_t<128, 32, 4, true>(S, grid, s);
+    else if (c.bm == 256 && c.bn == 32 && c.wm == 4) launch_tn_t<256, 32, 4, true>(S, grid, s);
+    else if (c.bm == 256 && c.bn == 64 && c.wm == 4) launch_tn_t<256, 64, 4, true>(S, grid, s);
+    else return fail(RT_ERR_ARG, "batch: no stage-1 TN kernel for tile %dx%d (%d waves along M)",
+                     c.bm, c.bn, c.wm);
+  } else if (c.bm == 32 && c.bn == 128 && c.wm == 1) launch_tn_t<32, 128, 1, false>(S, grid, s);
+  else if (c.bm == 64 && c.bn == 128 && c.wm == 2) launch_tn_t<64, 128, 2, false>(S, grid, s);
+  else if (c.bm == 64 && c.bn == 64 && c.wm == 2) launch_tn_t<64, 64, 2, false>(S, grid, s);
+  else if (c.bm == 128 && c.bn == 128 && c.wm == 2) launch_tn_t<128, 128, 2, false>(S, grid, s);
   else return fail(RT_ERR_ARG, "batch: no TN kernel for tile %dx%d (%d waves along M)", c.bm, c.bn,
                    c.wm);
   RTA_HIP(hipGetLastError());
@@ -619,7 +698,10 @@ static int launch_grouped(const GroupedStage& S, hipStream_t s) {
 
 static int run_stages(Batch* b, hipStream_t s) {
   RTA_TRY(launch_grouped(b->st[0], s));  // density
-  if (b->n_gpair > 0)
+  if (b->use_tn && b->n_gquad > 0)
+    hipLaunchKernelGGL(batch_gamma_x2_kernel<kGammaSlices>, dim3((unsigned)b->n_gquad),
+                       dim3(256), 0, s, b->d_mols, b->d_off_gquad, b->nmol);
+  else if (!b->use_tn && b->n_gpair > 0)
     hipLaunchKernelGGL(batch_gamma_kernel, dim3((unsigned)b->n_gpair), dim3(256), 0, s,
                        b->d_mols, b->d_off_gpair, b->nmol);
   if (b->n_j > 0)
@@ -631,13 +713,13 @@ static int run_stages(Batch* b, hipStream_t s) {
       hipLaunchKernelGGL(batch_cpad_kernel, dim3((unsigned)b->n_cpad), dim3(256), 0, s,
                          b->d_mols, b->d_off_cpad, b->nmol);
     RTA_HIP(hipGetLastError());
-    RTA_TRY(launch_tn(b->tn[1], s));  // ao2mo stage 1 -> X^T
-    RTA_TRY(launch_tn(b->tn[2], s));  // ao2mo stage 2 -> Bp
+    RTA_TRY(launch_tn(b->tn[1], s, true));  // ao2mo stage 1 -> X^T
+    RTA_TRY(launch_tn(b->tn[2], s, false));  // ao2mo stage 2 -> Bp
     if (b->n_bcopy > 0)
       hipLaunchKernelGGL(batch_bcopy_kernel, dim3((unsigned)b->n_bcopy), dim3(256), 0, s,
                          b->d_mols, b->d_off_bcopy, b->nmol);
     RTA_HIP(hipGetLastError());
-    RTA_TRY(launch_tn(b->tn[3], s));  // K partials from Bp
+    RTA_TRY(launch_tn(b->tn[3], s, false));  // K partials from Bp
   } else {
     RTA_TRY(launch_grouped(b->st[1], s));  // ao2mo stage 1
     RTA_TRY(launch_grouped(b->st[2], s));  // ao2mo stage 2
@@ -653,7 +735,7 @@ static int run_stages(Batch* b, hipStream_t s) {
 }
 
 static int launch_stage(Batch* b, int s, hipStream_t stream) {
-  if (b->use_tn && s >= 1) return launch_tn(b->tn[s], stream);
+  if (b->use_tn && s >= 1) return launch_tn(b->tn[s], stream, s == 1);
   return launch_grouped(b->st[s], stream);
 }
 
@@ -690,8 +772,9 @@ extern "C" int rt_batch_create(const rt_mol_desc* mols, int nmol, void** handle)
   const char* forced = getenv("RTA_BATCH_GROUPED");
   const bool use_tn = !(forced && atoi(forced) == 1);
   b->use_tn = use_tn;
-  int64_t ng = 0, nj = 0, nk = 0, nc = 0, nbc = 0, ngp = 0;
-  std::vector<int64_t> og(1, 0), oj(1, 0), ok(1, 0), oc(1, 0), obc(1, 0), ogp(1, 0);
+  int64_t ng = 0, nj = 0, nk = 0, nc = 0, nbc = 0, ngp = 0, ngq = 0;
+  std::vector<int64_t> og(1, 0), oj(1, 0), ok(1, 0), oc(1, 0), obc(1, 0), ogp(1, 0),
+      ogq(1, 0);
   for (int i = 0; i < nmol; ++i) {
     const rt_mol_desc& m = mols[i];
     if (!(m.nbas > 0 && m.naux > 0 && m.nocc > 0 && m.nocc <= m.nbas && m.ri3ao && m.C &&
@@ -734,13 +817,15 @@ extern "C" int rt_batch_create(const rt_mol_desc* mols, int nmol, void** handle)
     b->mols.push_back(d);
     const int64_t t32 = (m.nbas + 31) / 32;
     ng += m.naux;
-    nj += m.nbas * ((m.nbas + 63) / 64);
+    nj += (m.nbas * (m.nbas + 1) / 2 + 255) / 256;
     nk += t32 * (t32 + 1) / 2;
     nc += m.nbas;
     nbc += m.nbas * ((m.naux + 31) / 32);
     obc.push_back(nbc);
     ngp += (m.naux + 1) / 2;
     ogp.push_back(ngp);
+    ngq += (m.naux + kGammaSlices - 1) / kGammaSlices;
+    ogq.push_back(ngq);
     og.push_back(ng);
     oj.push_back(nj);
     ok.push_back(nk);
@@ -752,6 +837,7 @@ extern "C" int rt_batch_create(const rt_mol_desc* mols, int nmol, void** handle)
   b->n_cpad = nc;
   b->n_bcopy = nbc;
   b->n_gpair = ngp;
+  b->n_gquad = ngq;
 
   int st = dalloc(b, sizeof(MolDev) * nmol, (void**)&b->d_mols);
   if (st == RT_OK) st = dalloc(b, sizeof(int64_t) * og.size(), (void**)&b->d_off_gamma);
@@ -760,6 +846,7 @@ extern "C" int rt_batch_create(const rt_mol_desc* mols, int nmol, void** handle)
   if (st == RT_OK) st = dalloc(b, sizeof(int64_t) * oc.size(), (void**)&b->d_off_cpad);
   if (st == RT_OK) st = dalloc(b, sizeof(int64_t) * obc.size(), (void**)&b->d_off_bcopy);
   if (st == RT_OK) st = dalloc(b, sizeof(int64_t) * ogp.size(), (void**)&b->d_off_gpair);
+  if (st == RT_OK) st = dalloc(b, sizeof(int64_t) * ogq.size(), (void**)&b->d_off_gquad);
   if (st == RT_OK) st = dalloc(b, sizeof(double) * 2 * nmol, (void**)&b->energies);
   if (st != RT_OK) return fail_cleanup(st);
   if (hipMemcpy(b->d_mols, b->mols.data(), sizeof(MolDev) * nmol, hipMemcpyHostToDevice) !=
@@ -775,6 +862,8 @@ extern "C" int rt_batch_create(const rt_mol_desc* mols, int nmol, void** handle)
       hipMemcpy(b->d_off_bcopy, obc.data(), sizeof(int64_t) * obc.size(),
                 hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(b->d_off_gpair, ogp.data(), sizeof(int64_t) * ogp.size(),
+                hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(b->d_off_gquad, ogq.data(), sizeof(int64_t) * ogq.size(),
                 hipMemcpyHostToDevice) != hipSuccess)
     return fail_cleanup(fail(RT_ERR_HIP, "batch: descriptor upload failed"));
   for (int s = 0; s < 4; ++s) {

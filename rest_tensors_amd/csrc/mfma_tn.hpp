@@ -102,7 +102,7 @@ __device__ __forceinline__ void tn_issue(const double* __restrict__ base,
 
 // One work item w (tile x batch chunk) of a TN GEMM; all 256 threads call it.
 // smem: NSTAGE (BM + BN) BK doubles.  WAVES_M of the 4 waves along M.
-template <int BM, int BN, int BK, int NSTAGE, int WAVES_M = 2>
+template <int BM, int BN, int BK, int NSTAGE, int WAVES_M = 2, bool COLSKIP = false>
 __device__ __forceinline__ void tn_tile(const TnParams& g, int64_t w, double* __restrict__ smem) {
   static_assert(BK == 16 || BK == 32, "BK");
   constexpr int WAVES_N = 4 / WAVES_M;
@@ -134,7 +134,16 @@ __device__ __forceinline__ void tn_tile(const TnParams& g, int64_t w, double* __
   // Upper mode: a wave tile strictly below the diagonal is never read.  Its
   // wave keeps loading and synchronising but issues no LDS reads or MFMAs,
   // which leaves its SIMD's MFMA pipe to the co-resident workgroup.
-  const bool idle = g.upper && (m0 + wave_m * WM >= n0 + wave_n * WN + WN);
+  // COLSKIP: the 16-column MFMA blocks of the wave that lie wholly at or past
+  // N (a narrow N in a wide tile) get no fragment reads and no MFMAs; a wave
+  // with none left idles like the one below the diagonal.
+  int fn_act = FN;
+  if constexpr (COLSKIP) {
+    const int64_t left = g.N - (n0 + wave_n * WN);
+    fn_act = left <= 0 ? 0 : (left >= WN ? FN : (int)((left + 15) / 16));
+  }
+  const bool idle =
+      (g.upper && (m0 + wave_m * WM >= n0 + wave_n * WN + WN)) || (COLSKIP && fn_act == 0);
 
   f64x4 acc[FM][FN];
 #pragma unroll
@@ -199,6 +208,7 @@ __device__ __forceinline__ void tn_tile(const TnParams& g, int64_t w, double* __
       }
 #pragma unroll
       for (int j = 0; j < FN; ++j) {
+        if (COLSKIP && j >= fn_act) continue;
         const int r = wave_n * WN + j * 16 + (lane & 15);
         b[set][j] = *(const f64x2*)(Bs + r * BK + 2 * (c ^ tn_swz<BK>(r)));
       }
@@ -212,9 +222,11 @@ __device__ __forceinline__ void tn_tile(const TnParams& g, int64_t w, double* __
 #pragma unroll
         for (int i = 0; i < FM; ++i)
 #pragma unroll
-          for (int j = 0; j < FN; ++j)
+          for (int j = 0; j < FN; ++j) {
+            if (COLSKIP && j >= fn_act) continue;
             acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[kp & 1][j][s], a[kp & 1][i][s],
                                                              acc[i][j], 0, 0, 0);
+          }
       __builtin_amdgcn_sched_barrier(0);
     }
   }
@@ -252,7 +264,7 @@ __global__ __launch_bounds__(256) void dgemm_tn_kernel(TnParams g) {
 
 // Persistent grouped TN GEMM over a work list: descs[d] has descs[d].nwork
 // items, offs the exclusive prefix sums (offs[ndesc] = total).
-template <int BM, int BN, int BK, int NSTAGE, int WAVES_M>
+template <int BM, int BN, int BK, int NSTAGE, int WAVES_M, bool COLSKIP = false>
 __global__ __launch_bounds__(256) void dgemm_grouped_tn_kernel(const TnParams* __restrict__ descs,
                                                                const int64_t* __restrict__ offs,
                                                                int ndesc) {
@@ -266,7 +278,7 @@ __global__ __launch_bounds__(256) void dgemm_grouped_tn_kernel(const TnParams* _
       else hi = mid;
     }
     const TnParams g = descs[lo];
-    tn_tile<BM, BN, BK, NSTAGE, WAVES_M>(g, gw - offs[lo], smem);
+    tn_tile<BM, BN, BK, NSTAGE, WAVES_M, COLSKIP>(g, gw - offs[lo], smem);
     __syncthreads();  // every wave is done with LDS before the next item's loads
   }
 }

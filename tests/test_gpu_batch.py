@@ -113,9 +113,11 @@ def test_batch_eager_autotune_and_graph(D):
                                      ((128, 48, 4), (64, 128, 2), (64, 64, 2))])
 def test_batch_tn_tiles(D, s1, s2, k):
     """Every TN tile of stages 1, 2 and K, forced, against the single-molecule
-    path; nbas and nocc not multiples of 16 (the padded internal layouts)."""
+    path; nbas and nocc not multiples of 16 (the padded internal layouts).
+    nocc = 8, 16, 12, 31, 37: stage 1 (N = nocc) runs 1, 2 and 3 of a wave's
+    16-column MFMA blocks, and whole waves past N (64-wide tiles) idle."""
     from rest_tensors_amd.batch import Batch
-    mols = make_mols(D, [67, 130, 99, 250], seed0=4200)
+    mols = make_mols(D, [67, 130, 99, 250, 300], seed0=4200)
     refs = [single(D, m) for m in mols]
     b = Batch(mols)
     b.set_tiles("ao2mo_s1", *s1)
