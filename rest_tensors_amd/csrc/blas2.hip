@@ -55,6 +55,40 @@ __global__ __launch_bounds__(256) void gemv_t_kernel(const double* __restrict__ 
   if (threadIdx.x == 0) part[c * gridDim.y + s] = acc;
 }
 
+// 'T' with 16-byte loads of A (lda even, A 16-byte aligned, slice even): a
+// lane takes row pairs (i, i+1), 4 pairs in flight; an odd last row alone
+__global__ __launch_bounds__(256) void gemv_t_x2_kernel(const double* __restrict__ A,
+                                                        int64_t lda, int64_t m,
+                                                        const double* __restrict__ x,
+                                                        int64_t incx, int64_t slice,
+                                                        double* __restrict__ part) {
+  __shared__ double red[17];
+  const int64_t c = blockIdx.x, s = blockIdx.y;
+  const int64_t i0 = s * slice, i1 = (i0 + slice < m) ? i0 + slice : m;
+  const double2* a2 = reinterpret_cast<const double2*>(A + c * lda);
+  double acc = 0.0, acc2 = 0.0;
+  int64_t i = i0 + 2 * threadIdx.x;
+  for (; i + 3 * 512 + 1 < i1; i += 4 * 512) {
+    double2 v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) v[u] = a2[(i + u * 512) >> 1];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int64_t r = i + u * 512;
+      acc += v[u].x * x[r * incx];
+      acc2 += v[u].y * x[(r + 1) * incx];
+    }
+  }
+  for (; i + 1 < i1; i += 512) {
+    const double2 v = a2[i >> 1];
+    acc += v.x * x[i * incx];
+    acc2 += v.y * x[(i + 1) * incx];
+  }
+  if (i < i1) acc += A[c * lda + i] * x[i * incx];
+  acc = b2_block_sum(acc + acc2, red);
+  if (threadIdx.x == 0) part[c * gridDim.y + s] = acc;
+}
+
 // y[c] = alpha * sum_s part[c * nsplit + s] + beta * y[c]
 __global__ void gemv_t_finish(const double* __restrict__ part, int nsplit, int64_t n,
                               double alpha, double beta, double* __restrict__ y,
@@ -80,6 +114,49 @@ __global__ __launch_bounds__(256) void gemv_n_kernel(const double* __restrict__ 
   double acc = 0.0;
   for (int64_t c = c0; c < c1; ++c) acc += A[i + c * lda] * x[c * incx];
   part[s * m + i] = acc;
+}
+
+// 'N' with 16-byte loads of A (lda even, A 16-byte aligned): a lane owns the
+// row pair (i, i+1) (an odd last row alone), 8 columns in flight
+__global__ __launch_bounds__(256) void gemv_n_x2_kernel(const double* __restrict__ A,
+                                                        int64_t lda, int64_t m, int64_t n,
+                                                        const double* __restrict__ x,
+                                                        int64_t incx, int64_t slice,
+                                                        double* __restrict__ part) {
+  const int64_t i = 2 * (blockIdx.x * (int64_t)256 + threadIdx.x);
+  const int64_t s = blockIdx.y;
+  if (i >= m) return;
+  const int64_t c0 = s * slice, c1 = (c0 + slice < n) ? c0 + slice : n;
+  const double* a = A + i;
+  int64_t c = c0;
+  if (i + 1 < m) {
+    double ax = 0.0, ay = 0.0, bx = 0.0, by = 0.0;
+    for (; c + 7 < c1; c += 8) {
+      double2 v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v[u] = *reinterpret_cast<const double2*>(a + (c + u) * lda);
+#pragma unroll
+      for (int u = 0; u < 8; u += 2) {
+        const double xa = x[(c + u) * incx], xb = x[(c + u + 1) * incx];
+        ax += v[u].x * xa;
+        ay += v[u].y * xa;
+        bx += v[u + 1].x * xb;
+        by += v[u + 1].y * xb;
+      }
+    }
+    for (; c < c1; ++c) {
+      const double2 v = *reinterpret_cast<const double2*>(a + c * lda);
+      const double xc = x[c * incx];
+      ax += v.x * xc;
+      ay += v.y * xc;
+    }
+    part[s * m + i] = ax + bx;
+    part[s * m + i + 1] = ay + by;
+  } else {
+    double acc = 0.0;
+    for (; c < c1; ++c) acc += a[c * lda] * x[c * incx];
+    part[s * m + i] = acc;
+  }
 }
 
 __global__ void gemv_n_finish(const double* __restrict__ part, int nsplit, int64_t m,
@@ -140,29 +217,38 @@ extern "C" int rt_dgemv_dev(char trans, int64_t m, int64_t n, double alpha,
   RTA_CHECK(y && (x || lx == 0) && (A || m * n == 0), "NULL argument");
   hipStream_t s = (hipStream_t)stream;
   KernelTimer kt("dgemv", s);
+  // 16-byte loads of A: every column starts 16-byte aligned
+  const bool x2 = lda % 2 == 0 && ((uintptr_t)A & 15) == 0;
   if (t) {
     // columns of A are the outputs: split long columns so that >= 2048 blocks run
     const int nsplit = (int)std::max<int64_t>(
         1, std::min<int64_t>((m + 4095) / 4096, (2048 + n - 1) / n));
-    const int64_t slice = (m + nsplit - 1) / nsplit;
+    const int64_t slice = ((m + nsplit - 1) / nsplit + 1) / 2 * 2;  // even: x2 row pairs
     double* part = nullptr;
     RTA_TRY(scratch_get(SCRATCH_JK, sizeof(double) * n * nsplit, (void**)&part));
-    if (m > 0)
+    if (m > 0 && x2)
+      hipLaunchKernelGGL(gemv_t_x2_kernel, dim3((unsigned)n, nsplit), dim3(256), 0, s, A, lda,
+                         m, x, incx, slice, part);
+    else if (m > 0)
       hipLaunchKernelGGL(gemv_t_kernel, dim3((unsigned)n, nsplit), dim3(256), 0, s, A, lda,
                          m, x, incx, slice, part);
     hipLaunchKernelGGL(gemv_t_finish, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
                        part, m > 0 ? nsplit : 0, n, alpha, beta, y, incy);
   } else {
-    const int64_t rblocks = (m + 255) / 256;
+    const int64_t rows_per_block = x2 ? 512 : 256;
+    const int64_t rblocks = (m + rows_per_block - 1) / rows_per_block;
     const int nsplit = (int)std::max<int64_t>(
         1, std::min<int64_t>(std::min<int64_t>(n, 64), (2048 + rblocks - 1) / rblocks));
     const int64_t slice = (n + nsplit - 1) / nsplit;
     double* part = nullptr;
     RTA_TRY(scratch_get(SCRATCH_JK, sizeof(double) * m * nsplit, (void**)&part));
-    if (n > 0)
+    if (n > 0 && x2)
+      hipLaunchKernelGGL(gemv_n_x2_kernel, dim3((unsigned)rblocks, nsplit), dim3(256), 0, s, A,
+                         lda, m, n, x, incx, slice, part);
+    else if (n > 0)
       hipLaunchKernelGGL(gemv_n_kernel, dim3((unsigned)rblocks, nsplit), dim3(256), 0, s, A,
                          lda, m, n, x, incx, slice, part);
-    hipLaunchKernelGGL(gemv_n_finish, dim3((unsigned)rblocks), dim3(256), 0, s, part,
+    hipLaunchKernelGGL(gemv_n_finish, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, part,
                        n > 0 ? nsplit : 0, m, alpha, beta, y, incy);
   }
   RTA_HIP(hipGetLastError());

@@ -21,13 +21,17 @@ def relf(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
 
-@pytest.mark.parametrize("shape", [(1, 1), (37, 23), (5000, 3), (3, 5000), (70000, 40)])
+@pytest.mark.parametrize("pad", [3, 4])
+@pytest.mark.parametrize("shape", [(1, 1), (2, 3), (37, 23), (5000, 3), (3, 5000), (70000, 40),
+                                   (4099, 17), (1030, 9)])
 @pytest.mark.parametrize("trans", ["N", "T"])
-def test_dgemv(D, shape, trans):
+def test_dgemv(D, shape, trans, pad):
+    """lda = m + pad: an even lda takes the 16-byte kernels (odd m leaves one
+    row alone), an odd one the 8-byte kernels."""
     m, n = shape
     rng = np.random.default_rng(m * 7 + n)
     A = np.asfortranarray(rng.standard_normal((m, n)))
-    lda = m + 3
+    lda = m + pad
     Ap = np.zeros((lda, n), order="F")
     Ap[:m] = A
     for incx, incy, beta in ((1, 1, 0.0), (2, 3, -0.5)):
