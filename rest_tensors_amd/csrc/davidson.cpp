@@ -7,12 +7,22 @@
 // 3*(nroots-1) (:74), fresh restarts when space + nroots > maxspace (:260),
 // at most 40 new vectors (:115-117), the convergence test |de| < tol and
 // |r| < sqrt(tol) (:189), the diagonal preconditioner with the shift e[0]
-// for every root (:224), sequential projection against the subspace
-// (:235-243) and the lindep drops (:268-325).  The "large |r|" restore
-// (:210-217) compares against a value that is never updated (:259 shadows
-// it); the same constant 1e9 is used here.  Where the reference keeps using
-// a pre-QR vector count after _qr dropped vectors (:112-136, an out-of-range
-// index there), the post-QR count is used.
+// for every root (:224), projection against the subspace (:235-243) and the
+// lindep drops (:268-325).  The "large |r|" restore (:210-217) compares
+// against a value that is never updated (:259 shadows it); the same constant
+// 1e9 is used here.  Where the reference keeps using a pre-QR vector count
+// after _qr dropped vectors (:112-136, an out-of-range index there), the
+// post-QR count is used.
+//
+// Changed in form, equal in exact arithmetic: the projection of the new
+// vectors against the orthonormal subspace is t -= Q (Q^T t) done twice
+// (classical Gram-Schmidt with one reorthogonalization) instead of one
+// subspace vector at a time, and _qr runs its column-by-column loop with the
+// same drops on the small Gram matrix X^T X (host), then forms Q = X C on the
+// device, twice.  Both differ from the sequential order by rounding only and
+// use the whole GPU instead of one workgroup per vector.  The subspace
+// eigenproblem is solved by Householder tridiagonalization and implicit QL
+// (the reference calls LAPACK's dspevx).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -27,59 +37,59 @@
 
 namespace rta {
 
-// Cyclic Jacobi eigensolver for a small symmetric matrix (column-major,
-// n x n).  Eigenvalues ascending in w, eigenvectors in the columns of V.
-static void jacobi_eigh(const std::vector<double>& A_in, int n,
-                        std::vector<double>& w, std::vector<double>& V) {
-  std::vector<double> A(A_in);
+// Eigenpairs of the small symmetric subspace matrix (column-major n x n):
+// Householder reduction to tridiagonal form with Q accumulated (for k = 0..n-3
+// the reflector v = x - alpha e1 zeroes A[k+2:, k], alpha = -sign(x0) |x|),
+// then the implicit QL/QR iteration of rt_dsteqr_host applied to Q.
+// Eigenvalues ascending in w, eigenvectors in the columns of V.
+static int small_eigh(const std::vector<double>& A_in, int n, std::vector<double>& w,
+                      std::vector<double>& V) {
+  std::vector<double> A(A_in), v(n), p(n);
+  auto a = [&](int i, int j) -> double& { return A[i + (size_t)n * j]; };
   V.assign((size_t)n * n, 0.0);
   for (int i = 0; i < n; ++i) V[i + (size_t)n * i] = 1.0;
-  auto a = [&](int i, int j) -> double& { return A[i + (size_t)n * j]; };
-  for (int sweep = 0; sweep < 100; ++sweep) {
-    double off = 0.0, tot = 0.0;
-    for (int j = 0; j < n; ++j)
-      for (int i = 0; i < n; ++i) {
-        tot += a(i, j) * a(i, j);
-        if (i != j) off += a(i, j) * a(i, j);
-      }
-    if (off <= 1e-30 * tot || off == 0.0) break;
-    for (int p = 0; p < n - 1; ++p)
-      for (int q = p + 1; q < n; ++q) {
-        const double apq = a(p, q);
-        if (apq == 0.0) continue;
-        const double theta = (a(q, q) - a(p, p)) / (2.0 * apq);
-        const double t = (theta >= 0 ? 1.0 : -1.0) /
-                         (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
-        const double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
-        for (int k = 0; k < n; ++k) {  // columns p, q
-          const double akp = a(k, p), akq = a(k, q);
-          a(k, p) = c * akp - s * akq;
-          a(k, q) = s * akp + c * akq;
-        }
-        for (int k = 0; k < n; ++k) {  // rows p, q
-          const double apk = a(p, k), aqk = a(q, k);
-          a(p, k) = c * apk - s * aqk;
-          a(q, k) = s * apk + c * aqk;
-        }
-        for (int k = 0; k < n; ++k) {
-          const double vkp = V[k + (size_t)n * p], vkq = V[k + (size_t)n * q];
-          V[k + (size_t)n * p] = c * vkp - s * vkq;
-          V[k + (size_t)n * q] = s * vkp + c * vkq;
-        }
-      }
+  for (int k = 0; k + 2 < n; ++k) {
+    double xn = 0.0;
+    for (int i = k + 1; i < n; ++i) xn += a(i, k) * a(i, k);
+    xn = std::sqrt(xn);
+    if (xn == 0.0) continue;
+    const double x0 = a(k + 1, k);
+    const double alpha = x0 >= 0.0 ? -xn : xn;
+    // v = x - alpha e1, scaled to |v| = 1 (|v|^2 = 2 xn (xn + |x0|))
+    const double vn = std::sqrt(2.0 * xn * (xn + std::fabs(x0)));
+    for (int i = k + 1; i < n; ++i) v[i] = a(i, k) / vn;
+    v[k + 1] = (x0 - alpha) / vn;
+    // trailing block B = A[k+1:, k+1:] -> H B H, H = I - 2 v v^T:
+    // p = B v, q = p - (v.p) v, B -= 2 (v q^T + q v^T)
+    double vp = 0.0;
+    for (int i = k + 1; i < n; ++i) {
+      double acc = 0.0;
+      for (int j = k + 1; j < n; ++j) acc += a(i, j) * v[j];
+      p[i] = acc;
+    }
+    for (int i = k + 1; i < n; ++i) vp += v[i] * p[i];
+    for (int i = k + 1; i < n; ++i) p[i] -= vp * v[i];
+    for (int j = k + 1; j < n; ++j)
+      for (int i = k + 1; i < n; ++i) a(i, j) -= 2.0 * (v[i] * p[j] + p[i] * v[j]);
+    a(k + 1, k) = a(k, k + 1) = alpha;
+    for (int i = k + 2; i < n; ++i) a(i, k) = a(k, i) = 0.0;
+    // Q = Q H on columns k+1..n-1
+    for (int r = 0; r < n; ++r) {
+      double acc = 0.0;
+      for (int j = k + 1; j < n; ++j) acc += V[r + (size_t)n * j] * v[j];
+      for (int j = k + 1; j < n; ++j) V[r + (size_t)n * j] -= 2.0 * acc * v[j];
+    }
   }
-  std::vector<int> idx(n);
-  for (int i = 0; i < n; ++i) idx[i] = i;
-  std::stable_sort(idx.begin(), idx.end(),
-                   [&](int x, int y) { return a(x, x) < a(y, y); });
-  std::vector<double> V2((size_t)n * n);
+  std::vector<double> e(n > 1 ? n - 1 : 1, 0.0);
   w.resize(n);
-  for (int k = 0; k < n; ++k) {
-    w[k] = a(idx[k], idx[k]);
-    for (int i = 0; i < n; ++i)
-      V2[i + (size_t)n * k] = V[i + (size_t)n * idx[k]];
-  }
-  V.swap(V2);
+  for (int i = 0; i < n; ++i) w[i] = a(i, i);
+  for (int i = 0; i + 1 < n; ++i) e[i] = a(i + 1, i);
+  int info = 0;
+  RTA_TRY(rt_dsteqr_host('V', n, w.data(), e.data(), V.data(), n, &info));
+  if (info != 0)
+    return fail(RT_ERR_NUMERIC, "davidson: subspace eigensolver did not converge (info %d)",
+                info);
+  return RT_OK;
 }
 
 struct DevBuf {
@@ -131,7 +141,7 @@ extern "C" int rt_davidson_solve_dev(rt_matvec_fn a_x, void* ctx,
   const int cap = maxspace + 2 * nroots + 48;
   const int tmax = std::max(nroots, 40);
 
-  DevBuf XS, AX, X0, AX0, R, T, Sa, work, scr, small;
+  DevBuf XS, AX, X0, AX0, R, T, Sa, scr, small;
   RTA_TRY(XS.alloc((size_t)n * cap));
   RTA_TRY(AX.alloc((size_t)n * cap));
   RTA_TRY(X0.alloc((size_t)n * cap));
@@ -139,24 +149,23 @@ extern "C" int rt_davidson_solve_dev(rt_matvec_fn a_x, void* ctx,
   RTA_TRY(R.alloc((size_t)n * tmax));
   RTA_TRY(T.alloc((size_t)n * tmax));
   if (prm->precond == RT_PRECOND_MLP && prm->aux_op) RTA_TRY(Sa.alloc((size_t)n * tmax));
-  RTA_TRY(work.alloc((size_t)n));
   const int64_t scr_n =
       std::max<int64_t>(dv_gram_scratch(n, cap, tmax), dv_gram_scratch(n, 1, 1) * tmax);
   RTA_TRY(scr.alloc((size_t)scr_n));
-  // small device area: e (cap), shift (cap), scale (cap), sq (cap), coef (cap*cap),
-  // gram out (cap*tmax), int count
+  // small device area: e (cap), shift (cap), scale (cap), sq (cap), coef (cap*cap,
+  // also the _qr coefficients), gram out (cap*tmax, also the _qr and projection Gram
+  // matrices)
   const size_t o_e = 0, o_shift = cap, o_scale = 2 * (size_t)cap,
                o_sq = 3 * (size_t)cap, o_coef = 4 * (size_t)cap,
                o_gram = o_coef + (size_t)cap * cap,
-               o_cnt = o_gram + (size_t)cap * tmax;
-  RTA_TRY(small.alloc(o_cnt + 2));
+               o_end = o_gram + (size_t)cap * tmax;
+  RTA_TRY(small.alloc(o_end));
   double* d_e = small.p + o_e;
   double* d_shift = small.p + o_shift;
   double* d_scale = small.p + o_scale;
   double* d_sq = small.p + o_sq;
   double* d_coef = small.p + o_coef;
   double* d_gram = small.p + o_gram;
-  int* d_cnt = (int*)(small.p + o_cnt);
 
   RTA_HIP(hipMemcpyAsync(X0.p, x0_dev, sizeof(double) * n,
                          hipMemcpyDeviceToDevice, s));
@@ -167,7 +176,7 @@ extern "C" int rt_davidson_solve_dev(rt_matvec_fn a_x, void* ctx,
   std::vector<double> heff, w, V, e, elast;
   std::vector<int> conv(nroots, 0);
   std::vector<double> dx_norm, de, h_sq(cap), h_gram((size_t)cap * tmax),
-      h_coef((size_t)cap * cap), h_shift(cap), h_scale(cap);
+      h_coef((size_t)cap * cap), h_shift(cap), h_scale(cap), h_qr((size_t)tmax * tmax);
   int nroots_current = 0;
   rt_davidson_stats st;
   memset(&st, 0, sizeof st);
@@ -177,16 +186,13 @@ extern "C" int rt_davidson_solve_dev(rt_matvec_fn a_x, void* ctx,
     int xtlen = 0;
     if (fresh) {
       space = 0;
-      RTA_TRY(dv_mgs(X0.p, nx0, n, lindep, XS.p, d_cnt, work.p, s));
-      RTA_HIP(hipMemcpyAsync(&xtlen, d_cnt, sizeof(int), hipMemcpyDeviceToHost, s));
-      RTA_HIP(hipStreamSynchronize(s));
+      RTA_TRY(dv_qr(X0.p, nx0, n, lindep, XS.p, &xtlen, d_gram, d_coef, scr.p, h_qr.data(), s));
       if (xtlen == 0)
         return fail(RT_ERR_NUMERIC, "No more linear independent basis found");
       ++st.restarts;
     } else if (nt > 1) {
-      RTA_TRY(dv_mgs(T.p, nt, n, lindep, XS.p + (size_t)space * n, d_cnt, work.p, s));
-      RTA_HIP(hipMemcpyAsync(&xtlen, d_cnt, sizeof(int), hipMemcpyDeviceToHost, s));
-      RTA_HIP(hipStreamSynchronize(s));
+      RTA_TRY(dv_qr(T.p, nt, n, lindep, XS.p + (size_t)space * n, &xtlen, d_gram, d_coef,
+                    scr.p, h_qr.data(), s));
       xtlen = std::min(xtlen, 40);
     } else {
       RTA_HIP(hipMemcpyAsync(XS.p + (size_t)space * n, T.p, sizeof(double) * n,
@@ -225,7 +231,7 @@ extern "C" int rt_davidson_solve_dev(rt_matvec_fn a_x, void* ctx,
       heff.swap(hn);
     }
     elast = e;
-    jacobi_eigh(heff, space, w, V);
+    RTA_TRY(small_eigh(heff, space, w, V));
     const int nkeep = std::min(space, nroots);
     e.assign(w.begin(), w.begin() + nkeep);
     nroots_current = nkeep;
@@ -331,7 +337,7 @@ extern "C" int rt_davidson_solve_dev(rt_matvec_fn a_x, void* ctx,
       RTA_HIP(hipMemcpyAsync(d_scale, h_scale.data(), sizeof(double) * nt,
                              hipMemcpyHostToDevice, s));
       RTA_TRY(dv_scale_cols(T.p, d_scale, nt, n, s));
-      RTA_TRY(dv_project_out(XS.p, n, space, T.p, nt, n, s));
+      RTA_TRY(dv_project_out_block(XS.p, n, space, T.p, nt, n, d_gram, scr.p, s));
       RTA_TRY(dv_sqnorms(T.p, nt, n, d_sq, scr.p, s));
       RTA_HIP(hipMemcpyAsync(h_sq.data(), d_sq, sizeof(double) * nt,
                              hipMemcpyDeviceToHost, s));

@@ -9,6 +9,7 @@
 #include <vector>
 
 #include <algorithm>
+#include <cmath>
 
 #include "common.hpp"
 #include "dvkernels.hpp"
@@ -41,22 +42,54 @@ __device__ double dv_block_sum(double v, double* red) {
 }
 
 // ---- Gram blocks: G[i + ldg*j] = U[:, i] . W[:, j] ---------------------------
-// Split over x into `nsplit` slices; partial[s][i + nu*j]; then a fixed-order sum.
-__global__ __launch_bounds__(256) void gram_partial_kernel(
+// Block (slice of rows, 8 columns of U, 8 columns of W): every
+// thread sums its rows into an 8 x 8 register tile, the tiles are summed per
+// wave and across the 4 waves in a fixed order; partial[s][i + nu*j] per
+// slice s, then a fixed-order sum over the slices.
+// rows per slice: at least kGramRows, at most kGramSlices slices
+constexpr int64_t kGramRows = 1024, kGramSlices = 512;
+
+__global__ __launch_bounds__(256) void gram_tile_kernel(
     const double* __restrict__ U, int64_t ldu, int nu,
     const double* __restrict__ W, int64_t ldw, int nw, int64_t n, int64_t slice,
     double* __restrict__ partial) {
-  __shared__ double red[40];
+  constexpr int TI = 8, TJ = 8;
+  __shared__ double red[4][TI * TJ];
   const int64_t x0 = (int64_t)blockIdx.x * slice;
   const int64_t x1 = (x0 + slice < n) ? x0 + slice : n;
-  for (int p = 0; p < nu * nw; ++p) {
-    const int i = p % nu, j = p / nu;
-    const double* u = U + (int64_t)i * ldu;
-    const double* w = W + (int64_t)j * ldw;
-    double acc = 0.0;
-    for (int64_t x = x0 + threadIdx.x; x < x1; x += 256) acc += u[x] * w[x];
-    acc = dv_block_sum(acc, red);
-    if (threadIdx.x == 0) partial[(int64_t)blockIdx.x * nu * nw + p] = acc;
+  const int i0 = blockIdx.y * TI, j0 = blockIdx.z * TJ;
+  const int ni = (nu - i0 < TI) ? nu - i0 : TI, nj = (nw - j0 < TJ) ? nw - j0 : TJ;
+  double acc[TI][TJ];
+#pragma unroll
+  for (int i = 0; i < TI; ++i)
+#pragma unroll
+    for (int j = 0; j < TJ; ++j) acc[i][j] = 0.0;
+  for (int64_t x = x0 + threadIdx.x; x < x1; x += 256) {
+    double u[TI], w[TJ];
+#pragma unroll
+    for (int i = 0; i < TI; ++i) u[i] = (i < ni) ? U[x + (int64_t)(i0 + i) * ldu] : 0.0;
+#pragma unroll
+    for (int j = 0; j < TJ; ++j) w[j] = (j < nj) ? W[x + (int64_t)(j0 + j) * ldw] : 0.0;
+#pragma unroll
+    for (int i = 0; i < TI; ++i)
+#pragma unroll
+      for (int j = 0; j < TJ; ++j) acc[i][j] += u[i] * w[j];
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int i = 0; i < TI; ++i)
+#pragma unroll
+    for (int j = 0; j < TJ; ++j) {
+      const double v = dv_wave_sum(acc[i][j]);
+      if (lane == 0) red[wave][i + TI * j] = v;
+    }
+  __syncthreads();
+  const int t = threadIdx.x;
+  if (t < TI * TJ) {
+    const int i = t % TI, j = t / TI;
+    if (i < ni && j < nj)
+      partial[(int64_t)blockIdx.x * nu * nw + (i0 + i) + (int64_t)nu * (j0 + j)] =
+          (red[0][t] + red[1][t]) + (red[2][t] + red[3][t]);
   }
 }
 
@@ -71,7 +104,7 @@ __global__ void gram_reduce_kernel(const double* __restrict__ partial,
 }
 
 static int gram_nsplit(int64_t n) {
-  return (int)std::max<int64_t>(1, std::min<int64_t>(256, (n + 2047) / 2048));
+  return (int)std::max<int64_t>(1, std::min<int64_t>(kGramSlices, (n + kGramRows - 1) / kGramRows));
 }
 
 int dv_gram(const double* U, int64_t ldu, int nu, const double* W, int64_t ldw,
@@ -81,8 +114,8 @@ int dv_gram(const double* U, int64_t ldu, int nu, const double* W, int64_t ldw,
   const int ns = gram_nsplit(n);
   const int64_t slice = (n + ns - 1) / ns;
   KernelTimer kt("dv_gram", s);
-  hipLaunchKernelGGL(gram_partial_kernel, dim3(ns), dim3(256), 0, s, U, ldu,
-                     nu, W, ldw, nw, n, slice, scratch);
+  hipLaunchKernelGGL(gram_tile_kernel, dim3(ns, (nu + 7) / 8, (nw + 7) / 8), dim3(256), 0, s,
+                     U, ldu, nu, W, ldw, nw, n, slice, scratch);
   hipLaunchKernelGGL(gram_reduce_kernel, dim3((nu * nw + 255) / 256),
                      dim3(256), 0, s, scratch, ns, nu, nw, G, ldg);
   RTA_HIP(hipGetLastError());
@@ -257,49 +290,93 @@ int dv_project_out(const double* Q, int64_t ldq, int ns, double* T, int nt,
   return RT_OK;
 }
 
-// Modified Gram-Schmidt with dropping (_qr, davidson.rs:287-325), one
-// workgroup: columns of X (ld n) are processed in order; accepted ones are
-// normalized and written to Q (ld n) consecutively; count -> *nout.
-__global__ __launch_bounds__(1024) void mgs_kernel(const double* __restrict__ X,
-                                                   int nx, int64_t n,
-                                                   double lindep,
-                                                   double* __restrict__ Q,
-                                                   int* __restrict__ nout,
-                                                   double* __restrict__ work) {
-  __shared__ double red[40];
-  int nv = 0;
-  for (int i = 0; i < nx; ++i) {
-    const double* xi = X + (int64_t)i * n;
-    for (int64_t x = threadIdx.x; x < n; x += blockDim.x) work[x] = xi[x];
-    __syncthreads();
-    for (int j = 0; j < nv; ++j) {
-      const double* q = Q + (int64_t)j * n;
-      double acc = 0.0;
-      for (int64_t x = threadIdx.x; x < n; x += blockDim.x) acc += q[x] * work[x];
-      const double prod = dv_block_sum(acc, red);
-      for (int64_t x = threadIdx.x; x < n; x += blockDim.x) work[x] -= q[x] * prod;
-      __syncthreads();
-    }
-    double acc = 0.0;
-    for (int64_t x = threadIdx.x; x < n; x += blockDim.x) acc += work[x] * work[x];
-    const double nn = dv_block_sum(acc, red);
-    const double norm = sqrt(nn);
-    if (norm * norm > lindep) {
-      double* q = Q + (int64_t)nv * n;
-      for (int64_t x = threadIdx.x; x < n; x += blockDim.x) q[x] = work[x] / norm;
-      ++nv;
-    }
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) *nout = nv;
+// T[:, c] -= sum_s Q[:, s] G[s + ns*c]
+__global__ __launch_bounds__(256) void sub_combine_kernel(const double* __restrict__ Q,
+                                                          int64_t ldq, int ns,
+                                                          const double* __restrict__ G,
+                                                          double* __restrict__ T, int64_t n) {
+  const int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int c = blockIdx.y;
+  if (x >= n) return;
+  double acc = 0.0;
+  for (int q = 0; q < ns; ++q) acc += Q[x + q * ldq] * G[q + (int64_t)ns * c];
+  T[x + c * n] -= acc;
 }
 
-int dv_mgs(const double* X, int nx, int64_t n, double lindep, double* Q,
-           int* nout_dev, double* work, hipStream_t s) {
+int dv_project_out_block(const double* Q, int64_t ldq, int ns, double* T, int nt, int64_t n,
+                         double* G, double* scratch, hipStream_t s) {
+  if (ns <= 0 || nt <= 0) return RT_OK;
+  KernelTimer kt("dv_project", s);
+  for (int pass = 0; pass < 2; ++pass) {
+    RTA_TRY(dv_gram(Q, ldq, ns, T, n, nt, n, G, ns, scratch, s));
+    hipLaunchKernelGGL(sub_combine_kernel, dim3((unsigned)((n + 255) / 256), nt), dim3(256), 0,
+                       s, Q, ldq, ns, G, T, n);
+    RTA_HIP(hipGetLastError());
+  }
+  return RT_OK;
+}
+
+// Modified Gram-Schmidt with drops on the Gram matrix G = X^T X (host,
+// k x k): column i becomes v = e_i - sum_j (c_j^T G v) c_j over the accepted
+// coefficient columns c_j in order, kept as v / |v|_G when v^T G v > lindep.
+// The accepted columns go to C (k x nv, ld k); returns nv.
+static int gs_gram_host(const std::vector<double>& G, int k, double lindep, double* C) {
+  std::vector<double> v(k), Gv(k);
+  int nv = 0;
+  for (int i = 0; i < k; ++i) {
+    for (int r = 0; r < k; ++r) v[r] = (r == i) ? 1.0 : 0.0;
+    for (int j = 0; j < nv; ++j) {
+      const double* cj = C + (size_t)k * j;
+      double prod = 0.0;
+      for (int c = 0; c < k; ++c) {
+        double gv = 0.0;
+        for (int r = 0; r < k; ++r) gv += G[c + (size_t)k * r] * v[r];
+        prod += cj[c] * gv;
+      }
+      for (int r = 0; r < k; ++r) v[r] -= prod * cj[r];
+    }
+    double nn = 0.0;
+    for (int c = 0; c < k; ++c) {
+      double gv = 0.0;
+      for (int r = 0; r < k; ++r) gv += G[c + (size_t)k * r] * v[r];
+      nn += v[c] * gv;
+    }
+    const double norm = std::sqrt(std::max(nn, 0.0));
+    if (norm * norm > lindep) {
+      double* cv = C + (size_t)k * nv;
+      for (int r = 0; r < k; ++r) cv[r] = v[r] / norm;
+      ++nv;
+    }
+  }
+  return nv;
+}
+
+int dv_qr(double* X, int nx, int64_t n, double lindep, double* Q, int* nout, double* G,
+          double* coef, double* scratch, double* host, hipStream_t s) {
   KernelTimer kt("dv_mgs", s);
-  hipLaunchKernelGGL(mgs_kernel, dim3(1), dim3(1024), 0, s, X, nx, n, lindep, Q,
-                     nout_dev, work);
-  RTA_HIP(hipGetLastError());
+  *nout = 0;
+  int k = nx;
+  std::vector<double> Gh;
+  for (int pass = 0; pass < 2 && k > 0; ++pass) {
+    // pass 0: X -> Q; pass 1: Q -> X, copied back to Q
+    const double* src = pass == 0 ? X : Q;
+    double* dst = pass == 0 ? Q : X;
+    RTA_TRY(dv_gram(src, n, k, src, n, k, n, G, k, scratch, s));
+    Gh.assign((size_t)k * k, 0.0);
+    RTA_HIP(hipMemcpyAsync(Gh.data(), G, sizeof(double) * k * k, hipMemcpyDeviceToHost, s));
+    RTA_HIP(hipStreamSynchronize(s));
+    // the synchronization above also retired the previous pass's upload and
+    // combine, so host and coef are free again
+    const int nv = gs_gram_host(Gh, k, lindep, host);
+    if (nv > 0) {
+      RTA_HIP(hipMemcpyAsync(coef, host, sizeof(double) * k * nv, hipMemcpyHostToDevice, s));
+      RTA_TRY(dv_combine(src, n, k, coef, k, nv, n, dst, n, s));
+      if (pass == 1)
+        RTA_HIP(hipMemcpyAsync(Q, X, sizeof(double) * n * nv, hipMemcpyDeviceToDevice, s));
+    }
+    k = nv;
+  }
+  *nout = k;
   return RT_OK;
 }
 

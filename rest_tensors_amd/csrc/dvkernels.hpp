@@ -61,7 +61,16 @@ int dv_scale_cols(double* X, const double* scale, int nk, int64_t n,
                   hipStream_t s);
 int dv_project_out(const double* Q, int64_t ldq, int ns, double* T, int nt,
                    int64_t n, hipStream_t s);
-int dv_mgs(const double* X, int nx, int64_t n, double lindep, double* Q,
-           int* nout_dev, double* work, hipStream_t s);
+// T -= Q (Q^T T), twice (ns + nt columns; G >= ns*nt doubles, scratch >=
+// dv_gram_scratch(n, ns, nt)); classical Gram-Schmidt with reorthogonalization
+int dv_project_out_block(const double* Q, int64_t ldq, int ns, double* T, int nt, int64_t n,
+                         double* G, double* scratch, hipStream_t s);
+// _qr (modified Gram-Schmidt with lindep drops) as Gram-matrix Gram-Schmidt,
+// run twice: the accepted columns of X (ld n) go to Q (ld n), their count to
+// *nout (host).  X is overwritten.  G, coef: device, >= nx^2 doubles;
+// scratch >= dv_gram_scratch(n, nx, nx); host: >= nx^2 doubles that stay
+// valid until the stream has passed this call.
+int dv_qr(double* X, int nx, int64_t n, double lindep, double* Q, int* nout, double* G,
+          double* coef, double* scratch, double* host, hipStream_t s);
 
 }  // namespace rta
