@@ -32,6 +32,7 @@ enum ScratchSlot {
   SCRATCH_JK,
   SCRATCH_DV_Z,
   SCRATCH_DV_YPART,
+  SCRATCH_DV_ZPART,
   SCRATCH_LOWP_A,
   SCRATCH_LOWP_B,
   SCRATCH_MLP_A,

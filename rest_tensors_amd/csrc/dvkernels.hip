@@ -463,6 +463,157 @@ __global__ __launch_bounds__(256) void skinny_b_z_kernel(
   }
 }
 
+// 16-byte forms of the two passes (ldb and n even, B and the vectors 16-byte
+// aligned): a lane takes the row pair (x, x+1).
+// Zpart[r][P + naux*c] = sum over rows x of slice r of B[x + ldb*P] X[x + n*c]
+// (8 aux columns per block, rows split into gridDim.y slices of `rows`)
+template <int KMAX>
+__global__ __launch_bounds__(256) void skinny_bt_x2_kernel(
+    const double* __restrict__ B, int64_t ldb, int64_t naux,
+    const double* __restrict__ X, int64_t n, int k, int64_t rows,
+    double* __restrict__ Zpart) {
+  constexpr int PG = 8;
+  __shared__ double red[PG * KMAX][5];
+  const int64_t P0 = (int64_t)blockIdx.x * PG;
+  const int64_t r0 = (int64_t)blockIdx.y * rows;
+  const int64_t r1 = (r0 + rows < n) ? r0 + rows : n;
+  double acc[PG][KMAX];
+#pragma unroll
+  for (int p = 0; p < PG; ++p)
+#pragma unroll
+    for (int c = 0; c < KMAX; ++c) acc[p][c] = 0.0;
+  for (int64_t x = r0 + 2 * threadIdx.x; x < r1; x += 512) {
+    double2 xv[KMAX];
+#pragma unroll
+    for (int c = 0; c < KMAX; ++c)
+      xv[c] = (c < k) ? *reinterpret_cast<const double2*>(X + x + n * c) : make_double2(0.0, 0.0);
+#pragma unroll
+    for (int p = 0; p < PG; ++p) {
+      const double2 b = (P0 + p < naux)
+                            ? *reinterpret_cast<const double2*>(B + x + ldb * (P0 + p))
+                            : make_double2(0.0, 0.0);
+#pragma unroll
+      for (int c = 0; c < KMAX; ++c) acc[p][c] += b.x * xv[c].x + b.y * xv[c].y;
+    }
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int p = 0; p < PG; ++p)
+#pragma unroll
+    for (int c = 0; c < KMAX; ++c) {
+      double v = dv_wave_sum(acc[p][c]);
+      if (lane == 0) red[p * KMAX + c][wave] = v;
+    }
+  __syncthreads();
+  for (int q = threadIdx.x; q < PG * KMAX; q += 256) {
+    const int p = q / KMAX, c = q % KMAX;
+    if (P0 + p < naux && c < k)
+      Zpart[(int64_t)blockIdx.y * naux * k + (P0 + p) + naux * c] =
+          (red[q][0] + red[q][1]) + (red[q][2] + red[q][3]);
+  }
+}
+
+// Z[e] = sum_r Zpart[r][e], e < naux*k, in slice order
+__global__ void skinny_zsum_kernel(const double* __restrict__ Zpart, int nr, int64_t m,
+                                   double* __restrict__ Z) {
+  const int64_t e = blockIdx.x * (int64_t)256 + threadIdx.x;
+  if (e >= m) return;
+  double s = 0.0;
+  for (int r = 0; r < nr; ++r) s += Zpart[r * m + e];
+  Z[e] = s;
+}
+
+// Ypart[s][x + n*c] = sum_{P in slice s} B[x + ldb*P] Z[P + naux*c], 8 slices
+// of B in flight per lane
+template <int KMAX>
+__global__ __launch_bounds__(256) void skinny_b_z2_kernel(
+    const double* __restrict__ B, int64_t ldb, int64_t naux,
+    const double* __restrict__ Z, int64_t n, int k, int64_t pslice,
+    double* __restrict__ Ypart) {
+  const int64_t x = 2 * (blockIdx.x * (int64_t)256 + threadIdx.x);
+  const int64_t s = blockIdx.y;
+  const int64_t P0 = s * pslice;
+  const int64_t P1 = (P0 + pslice < naux) ? P0 + pslice : naux;
+  if (x >= n) return;
+  double2 acc[KMAX];
+#pragma unroll
+  for (int c = 0; c < KMAX; ++c) acc[c] = make_double2(0.0, 0.0);
+  const double* b = B + x;
+  int64_t P = P0;
+  for (; P + 7 < P1; P += 8) {
+    double2 bv[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) bv[u] = *reinterpret_cast<const double2*>(b + ldb * (P + u));
+#pragma unroll
+    for (int c = 0; c < KMAX; ++c) {
+      if (c < k) {
+        const double* z = Z + naux * c + P;
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          acc[c].x += bv[u].x * z[u];
+          acc[c].y += bv[u].y * z[u];
+        }
+      }
+    }
+  }
+  for (; P < P1; ++P) {
+    const double2 bv = *reinterpret_cast<const double2*>(b + ldb * P);
+#pragma unroll
+    for (int c = 0; c < KMAX; ++c)
+      if (c < k) {
+        acc[c].x += bv.x * Z[naux * c + P];
+        acc[c].y += bv.y * Z[naux * c + P];
+      }
+  }
+#pragma unroll
+  for (int c = 0; c < KMAX; ++c)
+    if (c < k) *reinterpret_cast<double2*>(Ypart + s * n * k + x + n * c) = acc[c];
+}
+
+// The two passes over B for a block of kc <= 8 vectors: Z = B^T X, and the
+// kBzSlices partial sums Ypart[s] of B Z over slices s of the aux index.
+constexpr int kBzSlices = 16;   // partial slices of the second pass
+constexpr int kBtxRowSlices = 4;
+
+static bool skinny_x2(const double* B, int64_t ldb, const double* X, int64_t n) {
+  return ldb % 2 == 0 && n % 2 == 0 && ((uintptr_t)B & 15) == 0 && ((uintptr_t)X & 15) == 0;
+}
+
+static int skinny_btx(const double* B, int64_t ldb, int64_t naux, const double* X, int64_t n,
+                      int kc, double* Z, hipStream_t s) {
+  KernelTimer kt("casida_btx", s);
+  if (skinny_x2(B, ldb, X, n)) {
+    double* Zpart;
+    RTA_TRY(scratch_get(SCRATCH_DV_ZPART, sizeof(double) * kBtxRowSlices * naux * 8,
+                        (void**)&Zpart));
+    const int64_t rows = ((n + kBtxRowSlices - 1) / kBtxRowSlices + 1) / 2 * 2;
+    hipLaunchKernelGGL(skinny_bt_x2_kernel<8>,
+                       dim3((unsigned)((naux + 7) / 8), kBtxRowSlices), dim3(256), 0, s, B,
+                       ldb, naux, X, n, kc, rows, Zpart);
+    hipLaunchKernelGGL(skinny_zsum_kernel, dim3((unsigned)((naux * kc + 255) / 256)),
+                       dim3(256), 0, s, Zpart, kBtxRowSlices, naux * kc, Z);
+  } else {
+    hipLaunchKernelGGL(skinny_bt_x_kernel<8>, dim3((unsigned)((naux + 7) / 8)), dim3(256), 0,
+                       s, B, ldb, naux, X, n, kc, Z);
+  }
+  RTA_HIP(hipGetLastError());
+  return RT_OK;
+}
+
+static int skinny_bz(const double* B, int64_t ldb, int64_t naux, const double* Z, int64_t n,
+                     int kc, double* Ypart, hipStream_t s) {
+  KernelTimer kt("casida_bz", s);
+  const int64_t pslice = (naux + kBzSlices - 1) / kBzSlices;
+  if (skinny_x2(B, ldb, Ypart, n))
+    hipLaunchKernelGGL(skinny_b_z2_kernel<8>, dim3((unsigned)((n + 511) / 512), kBzSlices),
+                       dim3(256), 0, s, B, ldb, naux, Z, n, kc, pslice, Ypart);
+  else
+    hipLaunchKernelGGL(skinny_b_z_kernel<8>, dim3((unsigned)((n + 255) / 256), kBzSlices),
+                       dim3(256), 0, s, B, ldb, naux, Z, n, kc, pslice, Ypart);
+  RTA_HIP(hipGetLastError());
+  return RT_OK;
+}
+
 // Y[x + n*c] = d[x] X[x + n*c] + alpha * sum_s Ypart[s][x + n*c]
 __global__ __launch_bounds__(256) void skinny_finish_kernel(
     const double* __restrict__ Ypart, int nslice, const double* __restrict__ d,
@@ -503,23 +654,12 @@ int casida_apply(const CasidaOp* op, const double* X, double* Y, int64_t n,
     double* Yc = Y + c0 * n;
     double *Z, *Ypart;
     RTA_TRY(scratch_get(SCRATCH_DV_Z, sizeof(double) * op->naux * 8, (void**)&Z));
-    const int nslice = 8;
-    const int64_t pslice = (op->naux + nslice - 1) / nslice;
-    RTA_TRY(scratch_get(SCRATCH_DV_YPART, sizeof(double) * nslice * n * 8,
+    RTA_TRY(scratch_get(SCRATCH_DV_YPART, sizeof(double) * kBzSlices * n * 8,
                         (void**)&Ypart));
-    {
-      KernelTimer kt("casida_btx", s);
-      hipLaunchKernelGGL(skinny_bt_x_kernel<8>, dim3((unsigned)((op->naux + 7) / 8)),
-                         dim3(256), 0, s, op->B, op->ldb, op->naux, Xc, n, kc, Z);
-    }
-    {
-      KernelTimer kt("casida_bz", s);
-      hipLaunchKernelGGL(skinny_b_z_kernel<8>,
-                         dim3((unsigned)((n + 255) / 256), nslice), dim3(256), 0,
-                         s, op->B, op->ldb, op->naux, Z, n, kc, pslice, Ypart);
-    }
+    RTA_TRY(skinny_btx(op->B, op->ldb, op->naux, Xc, n, kc, Z, s));
+    RTA_TRY(skinny_bz(op->B, op->ldb, op->naux, Z, n, kc, Ypart, s));
     hipLaunchKernelGGL(skinny_finish_kernel, dim3((unsigned)((n * kc + 255) / 256)),
-                       dim3(256), 0, s, Ypart, nslice, op->d, Xc, alpha, n, kc,
+                       dim3(256), 0, s, Ypart, kBzSlices, op->d, Xc, alpha, n, kc,
                        Yc);
     RTA_HIP(hipGetLastError());
   }
@@ -693,24 +833,21 @@ int casida_si_apply(const CasidaSI* si, const double* R, double* T, int64_t n,
             (long long)op.n);
   if (k <= 0) return RT_OK;
   const int64_t m = op.naux;
-  const int nslice = 8;
-  const int64_t pslice = (m + nslice - 1) / nslice;
   double *G, *Z, *Ypart;
-  RTA_TRY(scratch_get(SCRATCH_DV_Z, sizeof(double) * (2 * m * 8 + n * 8), (void**)&Z));
+  // G (n x 8) first: 16-byte aligned like the scratch base, for the x2 passes
+  RTA_TRY(scratch_get(SCRATCH_DV_Z, sizeof(double) * (n * 8 + 2 * m * 8), (void**)&G));
+  Z = G + n * 8;
   double* W = Z + m * 8;
-  G = W + m * 8;
-  RTA_TRY(scratch_get(SCRATCH_DV_YPART, sizeof(double) * nslice * n * 8, (void**)&Ypart));
+  RTA_TRY(scratch_get(SCRATCH_DV_YPART, sizeof(double) * kBzSlices * n * 8, (void**)&Ypart));
   for (int64_t c0 = 0; c0 < k; c0 += 8) {
     const int kc = (int)std::min<int64_t>(8, k - c0);
     hipLaunchKernelGGL(si_diag_apply_kernel, dim3(si_grid(n * kc)), dim3(256), 0, s,
                        si->Dm, R + c0 * n, n, (int64_t)kc, G);
-    hipLaunchKernelGGL(skinny_bt_x_kernel<8>, dim3((unsigned)((m + 7) / 8)), dim3(256), 0,
-                       s, op.B, op.ldb, m, G, n, kc, Z);
+    RTA_TRY(skinny_btx(op.B, op.ldb, m, G, n, kc, Z, s));
     RTA_TRY(rt_dgemm_dev('T', 'N', m, kc, m, 1.0, si->Ci, m, Z, m, 0.0, W, m, s));
-    hipLaunchKernelGGL(skinny_b_z_kernel<8>, dim3((unsigned)((n + 255) / 256), nslice),
-                       dim3(256), 0, s, op.B, op.ldb, m, W, n, kc, pslice, Ypart);
+    RTA_TRY(skinny_bz(op.B, op.ldb, m, W, n, kc, Ypart, s));
     hipLaunchKernelGGL(si_finish_kernel, dim3(si_grid(n * kc)), dim3(256), 0, s, Ypart,
-                       nslice, si->Dm, G, n, kc, T + c0 * n);
+                       kBzSlices, si->Dm, G, n, kc, T + c0 * n);
     RTA_HIP(hipGetLastError());
   }
   return RT_OK;

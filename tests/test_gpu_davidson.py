@@ -54,6 +54,28 @@ def test_casida_matvec_and_diagonal(D, problem):
         assert np.linalg.norm(Y - ref) / np.linalg.norm(ref) < 1e-13
 
 
+@pytest.mark.parametrize("n,ldb,offset", [(302, 302, 0), (301, 303, 0), (302, 304, 1),
+                                          (4098, 4100, 0)])
+def test_casida_matvec_layouts(D, n, ldb, offset):
+    """A = diag(d) + 2 s^2 B B^T against the dense product for even strides
+    (16-byte passes) and odd or misaligned ones (8-byte passes)."""
+    from rest_tensors_amd import davidson as DV
+    rng = np.random.default_rng(n + ldb + offset)
+    naux, scale = 37, 0.05
+    Bfull = rng.standard_normal(offset + ldb * naux)
+    B = Bfull[offset:offset + ldb * naux].reshape((ldb, naux), order="F")[:n]
+    d = rng.uniform(0.5, 2.0, n)
+    dB, dd = D.DeviceArray.from_host(Bfull), D.DeviceArray.from_host(d)
+    op = DV.CasidaOp(dB, ldb, n, naux, scale, dd, offset=offset)
+    A = np.diag(d) + 2 * scale ** 2 * B @ B.T
+    for k in (1, 3, 8, 10):
+        X = rng.standard_normal((n, k))
+        dX, dY = D.DeviceArray.from_host(X), D.DeviceArray(n * k)
+        op.apply(dX, dY, k)
+        ref = A @ X
+        assert np.linalg.norm(dY.to_host((n, k)) - ref) / np.linalg.norm(ref) < 1e-13
+
+
 def x0_guess(D, hd_host):
     x0 = np.zeros(len(hd_host))
     x0[np.argmin(hd_host)] = 1.0
