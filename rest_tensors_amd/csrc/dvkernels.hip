@@ -844,7 +844,7 @@ int casida_si_apply(const CasidaSI* si, const double* R, double* T, int64_t n,
     hipLaunchKernelGGL(si_diag_apply_kernel, dim3(si_grid(n * kc)), dim3(256), 0, s,
                        si->Dm, R + c0 * n, n, (int64_t)kc, G);
     RTA_TRY(skinny_btx(op.B, op.ldb, m, G, n, kc, Z, s));
-    RTA_TRY(rt_dgemm_dev('T', 'N', m, kc, m, 1.0, si->Ci, m, Z, m, 0.0, W, m, s));
+    RTA_TRY(skinny_btx(si->Ci, m, m, Z, m, kc, W, s));  // W = Ci^T Z (Ci: m x m)
     RTA_TRY(skinny_bz(op.B, op.ldb, m, W, n, kc, Ypart, s));
     hipLaunchKernelGGL(si_finish_kernel, dim3(si_grid(n * kc)), dim3(256), 0, s, Ypart,
                        kBzSlices, si->Dm, G, n, kc, T + c0 * n);
