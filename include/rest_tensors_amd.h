@@ -372,15 +372,16 @@ int rt_casida_matvec(void* op, const double* X, double* Y, int64_t n,
 int rt_casida_hdiag(void* op, double* hdiag, void* stream);
 /* Shift-invert of the operator, T = (A - sigma)^-1 R with sigma = min(d) -
  * shift (shift > 0), by the Woodbury identity on the low-rank part; the
- * naux x naux capacitance inverse is built once (Newton-Schulz on FP64 MFMA
- * GEMMs).  rt_casida_si_apply is an rt_matvec_fn (aux_op of Davidson).
+ * naux x naux capacitance inverse is built once (FP64 MFMA GEMM, then a
+ * device Cholesky inverse).  rt_casida_si_apply is an rt_matvec_fn (aux_op
+ * of Davidson); rt_casida_si_info reports sigma and |I - C C^-1|_F.
  * Not in the reference: the structured correction of the learned
  * preconditioner. */
 int rt_casida_si_create(void* op, double shift, void** si, void* stream);
 int rt_casida_si_destroy(void* si);
 int rt_casida_si_apply(void* si, const double* R, double* T, int64_t n,
                        int64_t k, void* stream);
-int rt_casida_si_info(void* si, double* sigma, int* iters, double* err);
+int rt_casida_si_info(void* si, double* sigma, double* err);
 
 /* ---- learned preconditioner: 2-layer FP32 MLP (hidden 64) ------------------ */
 int rt_mlp_num_params(void);

@@ -93,7 +93,7 @@ RT_SIGNATURES = {
     "rt_casida_si_create": (i32, [vp, f64, C.POINTER(vp), vp]),
     "rt_casida_si_destroy": (i32, [vp]),
     "rt_casida_si_apply": (i32, [vp, vp, vp, i64, i64, vp]),
-    "rt_casida_si_info": (i32, [vp, pf64, pi32, pf64]),
+    "rt_casida_si_info": (i32, [vp, pf64, pf64]),
     "rt_mlp_num_params": (i32, []),
     "rt_mlp_create": (i32, [u64, C.POINTER(vp)]),
     "rt_mlp_destroy": (i32, [vp]),

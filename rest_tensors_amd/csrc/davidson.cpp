@@ -446,11 +446,10 @@ extern "C" int rt_casida_si_apply(void* si, const double* R, double* T, int64_t 
   return casida_si_apply((const CasidaSI*)si, R, T, n, k, (hipStream_t)stream);
 }
 
-extern "C" int rt_casida_si_info(void* si, double* sigma, int* iters, double* err) {
+extern "C" int rt_casida_si_info(void* si, double* sigma, double* err) {
   RTA_CHECK(si, "NULL argument");
   const CasidaSI* p = (const CasidaSI*)si;
   if (sigma) *sigma = p->sigma;
-  if (iters) *iters = p->ns_iters;
-  if (err) *err = p->ns_err;
+  if (err) *err = p->inv_err;
   return RT_OK;
 }

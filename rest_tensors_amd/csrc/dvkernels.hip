@@ -13,6 +13,8 @@
 
 #include "common.hpp"
 #include "dvkernels.hpp"
+#include "gemm.hpp"
+#include "metric.hpp"
 #include "timer.hpp"
 
 namespace rta {
@@ -697,25 +699,12 @@ __global__ void si_add_diag_kernel(double* __restrict__ M, int64_t m, double v) 
   if (i < m) M[i + m * i] += v;
 }
 
-// X = v * I (m x m), a fill kernel (no memset node in captured graphs)
-__global__ void si_scaled_eye_kernel(double* __restrict__ X, int64_t m, double v) {
-  const int64_t e = blockIdx.x * (int64_t)256 + threadIdx.x;
-  if (e >= m * m) return;
-  X[e] = (e % m == e / m) ? v : 0.0;
-}
-
 // E = I - T
 __global__ void si_eye_minus_kernel(const double* __restrict__ T, int64_t m,
                                     double* __restrict__ E) {
   const int64_t e = blockIdx.x * (int64_t)256 + threadIdx.x;
   if (e >= m * m) return;
   E[e] = ((e % m == e / m) ? 1.0 : 0.0) - T[e];
-}
-
-__global__ void si_copy_kernel(const double* __restrict__ a, int64_t len,
-                               double* __restrict__ b) {
-  const int64_t e = blockIdx.x * (int64_t)256 + threadIdx.x;
-  if (e < len) b[e] = a[e];
 }
 
 // G = Dm o R (n x k)
@@ -756,7 +745,8 @@ int casida_si_setup(CasidaSI* si, double shift, hipStream_t s) {
   RTA_HIP(hipMalloc(&si->Ci, sizeof(double) * m * m));
   hipLaunchKernelGGL(si_dm_kernel, dim3(si_grid(n)), dim3(256), 0, s, op.d, si->sigma,
                      n, si->Dm);
-  double *Bs = nullptr, *Cap = nullptr, *Xn = nullptr, *T = nullptr, *sq = nullptr;
+  double *Bs = nullptr, *Cap = nullptr, *Xn = nullptr, *T = nullptr, *sq = nullptr,
+         *part = nullptr;
   int st = RT_OK;
   auto alloc = [&](double** p, size_t cnt) {
     if (hipMalloc(p, sizeof(double) * cnt) != hipSuccess) {
@@ -773,47 +763,59 @@ int casida_si_setup(CasidaSI* si, double shift, hipStream_t s) {
   }
   double* X = si->Ci;
   double err = 0.0;
-  int it = 0;
   if (st == RT_OK) {
     // Cap = I/c + Bs^T Bs, Bs = sqrt(Dm) o B
     hipLaunchKernelGGL(si_scale_rows_kernel, dim3(si_grid(n * m)), dim3(256), 0, s, op.B,
                        op.ldb, si->Dm, n, m, Bs);
-    st = rt_dgemm_dev('T', 'N', m, m, n, 1.0, Bs, n, Bs, n, 0.0, Cap, m, s);
+    if (n % 16 == 0) {
+      // symmetric: the sum over 16-row blocks b of Bs_b^T Bs_b on upper tiles
+      // only, partial slots summed in a fixed order and mirrored
+      GemmDesc d;
+      d.ta = true;
+      d.tb = false;
+      d.M = d.N = m;
+      d.K = 16;
+      d.A = d.B = Bs;
+      d.lda = d.ldb = n;
+      d.sA = d.sB = 16;
+      d.batch = n / 16;
+      const int64_t bchunk = reduce_bchunk(m, 16, d.batch, 128);
+      const int64_t nch = (d.batch + bchunk - 1) / bchunk;
+      if (!alloc(&part, (size_t)nch * m * m))
+        st = fail(RT_ERR_NOMEM, "shift-invert setup: allocation failed");
+      else
+        st = gemm_reduce_sym(d, bchunk, part, Cap, m, s, "si_cap", 128);
+    } else {
+      st = rt_dgemm_dev('T', 'N', m, m, n, 1.0, Bs, n, Bs, n, 0.0, Cap, m, s);
+    }
   }
   if (st == RT_OK) {
     hipLaunchKernelGGL(si_add_diag_kernel, dim3(si_grid(m)), dim3(256), 0, s, Cap, m,
                        1.0 / c);
-    // Newton-Schulz: X0 = I/|Cap|_F, X <- 2X - X Cap X; E = I - Cap X squares each step
-    st = dv_sqnorms(Cap, 1, m * m, sq, sq + 1, s);
+    // Ci = Cap^-1 by Cholesky (Cap >= I/c is positive definite): Cap = L L^T
+    // in Xn, T = L^-1, Ci = T^T T
+    RTA_HIP(hipMemcpyAsync(Xn, Cap, sizeof(double) * m * m, hipMemcpyDeviceToDevice, s));
+    int info = 0;
+    st = chol_lower(Xn, m, &info, reinterpret_cast<int*>(sq), s);
+    if (st == RT_OK && info != 0)
+      st = fail(RT_ERR_NUMERIC, "shift-invert: Cholesky of I/c + B^T Dm B failed at %d", info);
+    if (st == RT_OK) st = tri_inverse_lower(Xn, m, T, s);
+    if (st == RT_OK) st = rt_dgemm_dev('T', 'N', m, m, m, 1.0, T, m, T, m, 0.0, X, m, s);
   }
   if (st == RT_OK) {
-    double fro2 = 0.0;
-    RTA_HIP(hipMemcpyAsync(&fro2, sq, sizeof(double), hipMemcpyDeviceToHost, s));
-    RTA_HIP(hipStreamSynchronize(s));
-    hipLaunchKernelGGL(si_scaled_eye_kernel, dim3(si_grid(m * m)), dim3(256), 0, s, X, m,
-                       1.0 / std::sqrt(fro2));
-    for (it = 0; it < 60 && st == RT_OK; ++it) {
-      // Cap and X are symmetric: T = Cap^T X, X_new = 2X - X^T T
-      st = rt_dgemm_dev('T', 'N', m, m, m, 1.0, Cap, m, X, m, 0.0, T, m, s);
-      if (st != RT_OK) break;
+    // check: |I - Cap Ci|_F
+    st = rt_dgemm_dev('T', 'N', m, m, m, 1.0, Cap, m, X, m, 0.0, T, m, s);
+    if (st == RT_OK) {
       hipLaunchKernelGGL(si_eye_minus_kernel, dim3(si_grid(m * m)), dim3(256), 0, s, T, m,
                          Xn);
       st = dv_sqnorms(Xn, 1, m * m, sq, sq + 1, s);
-      if (st != RT_OK) break;
+    }
+    if (st == RT_OK) {
       RTA_HIP(hipMemcpyAsync(&err, sq, sizeof(double), hipMemcpyDeviceToHost, s));
       RTA_HIP(hipStreamSynchronize(s));
       err = std::sqrt(err);
-      if (err < 1e-10) break;
-      hipLaunchKernelGGL(si_copy_kernel, dim3(si_grid(m * m)), dim3(256), 0, s, X, m * m,
-                         Xn);
-      st = rt_dgemm_dev('T', 'N', m, m, m, -1.0, X, m, T, m, 2.0, Xn, m, s);
-      if (st != RT_OK) break;
-      hipLaunchKernelGGL(si_copy_kernel, dim3(si_grid(m * m)), dim3(256), 0, s, Xn, m * m,
-                         X);
+      if (!(err < 1e-6)) st = fail(RT_ERR_NUMERIC, "shift-invert: |I - C Ci| = %.3e", err);
     }
-    if (st == RT_OK && !(err < 1e-6))
-      st = fail(RT_ERR_NUMERIC, "shift-invert: Newton-Schulz stalled at |I - C X| = %.3e",
-                err);
   }
   (void)hipStreamSynchronize(s);
   if (Bs) (void)hipFree(Bs);
@@ -821,8 +823,8 @@ int casida_si_setup(CasidaSI* si, double shift, hipStream_t s) {
   if (Xn) (void)hipFree(Xn);
   if (T) (void)hipFree(T);
   if (sq) (void)hipFree(sq);
-  si->ns_iters = it;
-  si->ns_err = err;
+  if (part) (void)hipFree(part);
+  si->inv_err = err;
   return st;
 }
 

@@ -25,15 +25,14 @@ int casida_hdiag(const CasidaOp* op, double* hdiag, hipStream_t s);
 // the Woodbury identity with A = D + c B B^T, c = 2 scale^2, Dm = 1/(d - sigma)
 // (sigma below min d):
 //   (A - sigma)^-1 = Dm - Dm B (I/c + B^T Dm B)^-1 B^T Dm.
-// Ci = (I/c + B^T Dm B)^-1 (naux x naux) is formed once, by Newton-Schulz
-// iteration on FP64 GEMMs.
+// Ci = (I/c + B^T Dm B)^-1 (naux x naux) is formed once: the FP64 GEMM for
+// B^T Dm B, then the inverse through its Cholesky factor.
 struct CasidaSI {
   CasidaOp op;
   double sigma;
   double* Dm;   // device, n
   double* Ci;   // device, naux x naux
-  int ns_iters; // Newton-Schulz iterations used
-  double ns_err; // |I - Cap Ci|_F at the end
+  double inv_err; // |I - Cap Ci|_F
 };
 
 int casida_si_setup(CasidaSI* si, double shift, hipStream_t s);

@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "gemm.hpp"
 #include "metric.hpp"
 #include "timer.hpp"
 
@@ -22,6 +23,17 @@ constexpr int CH_NB = 64;
 
 static unsigned mt_grid(int64_t len) { return (unsigned)((len + 255) / 256); }
 
+// Row r0 + t of the 64-column block of M (ld) at columns c0.. into out[64]:
+// every address is clamped into the nr x nc valid part so that all 64 loads
+// are issued before the first is waited for (a bounds branch per load would
+// serialize them); values outside the valid part are replaced by the caller.
+__device__ __forceinline__ void panel_row(const double* __restrict__ M, int64_t ld, int64_t r0,
+                                          int nr, int64_t c0, int nc, int t, double* out) {
+  const double* row = M + r0 + (t < nr ? t : nr - 1);
+#pragma unroll
+  for (int j = 0; j < CH_NB; ++j) out[j] = row[ld * (c0 + (j < nc ? j : nc - 1))];
+}
+
 // One workgroup of 64 threads factors W[k0:k0+kb, k0:k0+kb] (lower) in LDS.
 // info (device): first failing pivot (1-based) if still 0.
 __global__ __launch_bounds__(CH_NB) void potf2_kernel(double* __restrict__ W, int64_t ld,
@@ -29,8 +41,12 @@ __global__ __launch_bounds__(CH_NB) void potf2_kernel(double* __restrict__ W, in
                                                       int* __restrict__ info) {
   __shared__ double L[CH_NB][CH_NB + 1];
   const int t = threadIdx.x;
-  for (int j = 0; j < CH_NB; ++j)
-    L[t][j] = (t < kb && j < kb) ? W[(k0 + t) + ld * (k0 + j)] : (t == j ? 1.0 : 0.0);
+  {
+    double v[CH_NB];
+    panel_row(W, ld, k0, kb, k0, kb, t, v);
+#pragma unroll
+    for (int j = 0; j < CH_NB; ++j) L[t][j] = (t < kb && j < kb) ? v[j] : (t == j ? 1.0 : 0.0);
+  }
   for (int j = 0; j < kb; ++j) {
     __syncthreads();
     if (t == j) {
@@ -41,8 +57,19 @@ __global__ __launch_bounds__(CH_NB) void potf2_kernel(double* __restrict__ W, in
     __syncthreads();
     if (t > j && t < kb) L[t][j] /= L[j][j];
     __syncthreads();
-    if (t > j && t < kb)
-      for (int c = j + 1; c <= t; ++c) L[t][c] -= L[t][j] * L[c][j];
+    if (t > j && t < kb) {
+      // independent updates, 4 in flight (the LDS latency, not the FMAs, sets the pace)
+      const double ltj = L[t][j];
+      int c = j + 1;
+      for (; c + 3 <= t; c += 4) {
+        const double a0 = L[c][j], a1 = L[c + 1][j], a2 = L[c + 2][j], a3 = L[c + 3][j];
+        L[t][c] -= ltj * a0;
+        L[t][c + 1] -= ltj * a1;
+        L[t][c + 2] -= ltj * a2;
+        L[t][c + 3] -= ltj * a3;
+      }
+      for (; c <= t; ++c) L[t][c] -= ltj * L[c][j];
+    }
   }
   __syncthreads();
   if (t < kb)
@@ -59,17 +86,32 @@ __global__ __launch_bounds__(CH_NB) void trsm_rows_kernel(double* __restrict__ W
   __shared__ double xs[CH_NB][CH_NB + 1];
   const int t = threadIdx.x;
   const int64_t rb = r0 + blockIdx.x * (int64_t)CH_NB;
-  for (int j = 0; j < CH_NB; ++j) {
-    L[t][j] = (t < kb && j < kb) ? W[(k0 + t) + ld * (k0 + j)] : (t == j ? 1.0 : 0.0);
-    xs[t][j] = (rb + t < r1 && j < kb) ? W[(rb + t) + ld * (k0 + j)] : 0.0;
+  {
+    double v[CH_NB];
+    panel_row(W, ld, k0, kb, k0, kb, t, v);
+#pragma unroll
+    for (int j = 0; j < CH_NB; ++j) L[t][j] = (t < kb && j < kb) ? v[j] : (t == j ? 1.0 : 0.0);
+    const int nr = (int)((r1 - rb < CH_NB) ? r1 - rb : CH_NB);
+    panel_row(W, ld, rb, nr, k0, kb, t, v);
+#pragma unroll
+    for (int j = 0; j < CH_NB; ++j) xs[t][j] = (t < nr && j < kb) ? v[j] : 0.0;
   }
   __syncthreads();
   if (rb + t >= r1) return;
   for (int j = 0; j < kb; ++j) {
-    double v = xs[t][j];
-    for (int i = 0; i < j; ++i) v -= xs[t][i] * L[j][i];
-    xs[t][j] = v / L[j][j];
+    // four partial sums in flight (fixed order)
+    double v0 = xs[t][j], v1 = 0.0, v2 = 0.0, v3 = 0.0;
+    int i = 0;
+    for (; i + 3 < j; i += 4) {
+      v0 -= xs[t][i] * L[j][i];
+      v1 -= xs[t][i + 1] * L[j][i + 1];
+      v2 -= xs[t][i + 2] * L[j][i + 2];
+      v3 -= xs[t][i + 3] * L[j][i + 3];
+    }
+    for (; i < j; ++i) v0 -= xs[t][i] * L[j][i];
+    xs[t][j] = ((v0 + v1) + (v2 + v3)) / L[j][j];
   }
+#pragma unroll 8
   for (int j = 0; j < kb; ++j) W[(rb + t) + ld * (k0 + j)] = xs[t][j];
 }
 
@@ -84,20 +126,34 @@ __global__ __launch_bounds__(CH_NB) void trsm_cols_kernel(const double* __restri
   __shared__ double ys[CH_NB][CH_NB + 1];
   const int t = threadIdx.x;
   const int64_t cb = blockIdx.x * (int64_t)CH_NB;
-  for (int j = 0; j < CH_NB; ++j)
-    L[t][j] = (t < kb && j < kb) ? Lf[(k0 + t) + ldl * (k0 + j)] : (t == j ? 1.0 : 0.0);
-  // ys[c][j] = X[k0 + j, cb + c]: thread t reads row k0+t across the 64 columns
-  for (int c = 0; c < CH_NB; ++c)
-    ys[c][t] = (t < kb && cb + c < nc) ? X[(k0 + t) + ldx * (cb + c)] : 0.0;
+  {
+    double v[CH_NB];
+    panel_row(Lf, ldl, k0, kb, k0, kb, t, v);
+#pragma unroll
+    for (int j = 0; j < CH_NB; ++j) L[t][j] = (t < kb && j < kb) ? v[j] : (t == j ? 1.0 : 0.0);
+    // ys[c][j] = X[k0 + j, cb + c]: thread t reads row k0+t across the 64 columns
+    const int ncb = (int)((nc - cb < CH_NB) ? nc - cb : CH_NB);
+    panel_row(X, ldx, k0, kb, cb, ncb, t, v);
+#pragma unroll
+    for (int c = 0; c < CH_NB; ++c) ys[c][t] = (t < kb && c < ncb) ? v[c] : 0.0;
+  }
   __syncthreads();
   if (cb + t < nc) {
     for (int j = 0; j < kb; ++j) {
-      double v = ys[t][j];
-      for (int i = 0; i < j; ++i) v -= L[j][i] * ys[t][i];
-      ys[t][j] = v / L[j][j];
+      double v0 = ys[t][j], v1 = 0.0, v2 = 0.0, v3 = 0.0;
+      int i = 0;
+      for (; i + 3 < j; i += 4) {
+        v0 -= L[j][i] * ys[t][i];
+        v1 -= L[j][i + 1] * ys[t][i + 1];
+        v2 -= L[j][i + 2] * ys[t][i + 2];
+        v3 -= L[j][i + 3] * ys[t][i + 3];
+      }
+      for (; i < j; ++i) v0 -= L[j][i] * ys[t][i];
+      ys[t][j] = ((v0 + v1) + (v2 + v3)) / L[j][j];
     }
   }
   __syncthreads();
+#pragma unroll 8
   for (int c = 0; c < CH_NB; ++c)
     if (t < kb && cb + c < nc) X[(k0 + t) + ldx * (cb + c)] = ys[c][t];
 }
@@ -162,21 +218,111 @@ int chol_lower(double* W, int64_t n, int* info_h, int* info_d, hipStream_t s) {
   return RT_OK;
 }
 
-// X (ld n) <- L^-1 for the lower factor in Lf (ld n): block rows in order,
-// X[b,:] = L_bb^-1 (I_b - L[b, :k0] X[:k0, :])
+// Diagonal CH_NB blocks of X (ld n) <- their inverses L_bb^-1, one
+// workgroup per block: thread t forward-substitutes column t of the identity
+// (y_j = 0 for j < t; four partial sums in flight), the block is written row
+// by row.
+__global__ __launch_bounds__(CH_NB) void trinv_diag_kernel(const double* __restrict__ Lf,
+                                                           int64_t n, double* __restrict__ X) {
+  __shared__ double L[CH_NB][CH_NB + 1];
+  __shared__ double ys[CH_NB][CH_NB + 1];
+  const int t = threadIdx.x;
+  const int64_t k0 = (int64_t)blockIdx.x * CH_NB;
+  const int kb = (int)((n - k0 < CH_NB) ? n - k0 : CH_NB);
+  {
+    double v[CH_NB];
+    panel_row(Lf, n, k0, kb, k0, kb, t, v);
+#pragma unroll
+    for (int j = 0; j < CH_NB; ++j) L[t][j] = (t < kb && j < kb) ? v[j] : (t == j ? 1.0 : 0.0);
+  }
+  __syncthreads();
+  for (int j = 0; j < kb; ++j) {
+    double v0 = (j == t) ? 1.0 : 0.0, v1 = 0.0, v2 = 0.0, v3 = 0.0;
+    int i = 0;
+    for (; i + 3 < j; i += 4) {
+      v0 -= L[j][i] * ys[t][i];
+      v1 -= L[j][i + 1] * ys[t][i + 1];
+      v2 -= L[j][i + 2] * ys[t][i + 2];
+      v3 -= L[j][i + 3] * ys[t][i + 3];
+    }
+    for (; i < j; ++i) v0 -= L[j][i] * ys[t][i];
+    ys[t][j] = (j < t) ? 0.0 : ((v0 + v1) + (v2 + v3)) / L[j][j];
+  }
+  __syncthreads();
+  if (t < kb)
+    for (int c = 0; c < kb; ++c) X[(k0 + t) + n * (k0 + c)] = ys[c][t];
+}
+
+// X (ld n) <- L^-1 for the lower factor in Lf (ld n), by recursive doubling:
+// the diagonal CH_NB blocks are inverted in one launch; then for h = CH_NB,
+// 2 CH_NB, ... every pair of inverted h-blocks on the diagonal is joined,
+//   [L11 0; L21 L22]^-1 = [X11 0; -X22 (L21 X11) X22],
+// the pairs of one level as two strided-batched GEMMs (the last, shorter
+// pair on its own).  The strict upper triangle of X is zero.
 int tri_inverse_lower(const double* Lf, int64_t n, double* X, hipStream_t s) {
   hipLaunchKernelGGL(mt_fill_kernel, dim3(mt_grid(n * n)), dim3(256), 0, s, X, n * n, 0.0);
-  for (int64_t k0 = 0; k0 < n; k0 += CH_NB) {
-    const int kb = (int)std::min<int64_t>(CH_NB, n - k0);
-    const int64_t nc = k0 + kb;
-    hipLaunchKernelGGL(mt_eye_block_kernel, dim3(1), dim3(CH_NB), 0, s, X, n, k0, kb);
-    if (k0 > 0)
-      RTA_TRY(rt_dgemm_dev('N', 'N', kb, nc, k0, -1.0, Lf + k0, n, X, n, 1.0, X + k0, n, s));
-    hipLaunchKernelGGL(trsm_cols_kernel, dim3((unsigned)((nc + CH_NB - 1) / CH_NB)),
-                       dim3(CH_NB), 0, s, Lf, n, X, n, k0, kb, nc);
-    RTA_HIP(hipGetLastError());
+  hipLaunchKernelGGL(trinv_diag_kernel, dim3((unsigned)((n + CH_NB - 1) / CH_NB)), dim3(CH_NB),
+                     0, s, Lf, n, X);
+  RTA_HIP(hipGetLastError());
+  if (n <= CH_NB) return RT_OK;
+  // T: L21 X11 of every pair of a level, h x h each (ld h), the shorter last
+  // pair included
+  int64_t tlen = 0;
+  for (int64_t h = CH_NB; h < n; h *= 2) {
+    const int64_t np = n / (2 * h);
+    tlen = std::max(tlen, np * h * h);
+    if (np * 2 * h + h < n) tlen = std::max(tlen, h * h);
   }
-  return RT_OK;
+  double* T = nullptr;
+  if (hipMalloc(&T, sizeof(double) * (size_t)tlen) != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(RT_ERR_NOMEM, "triangular inverse: scratch allocation failed");
+  }
+  int st = RT_OK;
+  for (int64_t h = CH_NB; h < n && st == RT_OK; h *= 2) {
+    const int64_t np = n / (2 * h);  // full pairs
+    auto join = [&](int64_t off, int64_t h2, int64_t batch) -> int {
+      GemmDesc d1;  // T_p = L21 X11 (h2 x h)
+      d1.M = h2;
+      d1.N = h;
+      d1.K = h;
+      d1.A = Lf + (off + h) + n * off;
+      d1.lda = n;
+      d1.sA = 2 * h * (n + 1);
+      d1.B = X + off + n * off;
+      d1.ldb = n;
+      d1.sB = 2 * h * (n + 1);
+      d1.C = T;
+      d1.c_rs = 1;
+      d1.c_cs = h;
+      d1.sC = h * h;
+      d1.batch = batch;
+      RTA_TRY(gemm_batched(d1, s, "tri_inv"));
+      GemmDesc d2;  // X21 = -X22 T_p (h2 x h)
+      d2.M = h2;
+      d2.N = h;
+      d2.K = h2;
+      d2.A = X + (off + h) + n * (off + h);
+      d2.lda = n;
+      d2.sA = 2 * h * (n + 1);
+      d2.B = T;
+      d2.ldb = h;
+      d2.sB = h * h;
+      d2.C = X + (off + h) + n * off;
+      d2.c_rs = 1;
+      d2.c_cs = n;
+      d2.sC = 2 * h * (n + 1);
+      d2.alpha = -1.0;
+      d2.batch = batch;
+      return gemm_batched(d2, s, "tri_inv");
+    };
+    if (np > 0) st = join(0, h, np);
+    const int64_t off = np * 2 * h;
+    if (st == RT_OK && off + h < n) st = join(off, n - off - h, 1);
+  }
+  (void)hipStreamSynchronize(s);
+  (void)hipFree(T);
+  return st;
 }
 
 struct MtBufs {

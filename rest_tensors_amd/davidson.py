@@ -141,9 +141,9 @@ class ShiftInvert:
         check(lib().rt_casida_si_apply(self.handle, R.ptr, T.ptr, self.n, k, stream))
 
     def info(self) -> dict:
-        sig, it, err = C.c_double(), C.c_int(), C.c_double()
-        check(lib().rt_casida_si_info(self.handle, C.byref(sig), C.byref(it), C.byref(err)))
-        return {"sigma": sig.value, "newton_schulz_iters": it.value, "residual": err.value}
+        sig, err = C.c_double(), C.c_double()
+        check(lib().rt_casida_si_info(self.handle, C.byref(sig), C.byref(err)))
+        return {"sigma": sig.value, "residual": err.value}
 
     def __del__(self):
         try:
