@@ -210,6 +210,7 @@ def main():
 
     if not args.no_extras:
         extras["hbm_kernels"] = bench_hbm_kernels(D, dA, nbas, naux, args)
+        extras["metric"] = bench_metric(D, naux)
 
     del dA
     if not args.no_extras and args.batch_mols > 0:
@@ -321,6 +322,38 @@ def bench_hbm_kernels(D, dA, nbas, naux, args):
     del dU, dUp
     D.synchronize()
     log("[bench] hbm kernels: " + ", ".join(f"{k} {v['GB/s']} GB/s" for k, v in out.items()))
+    return out
+
+
+def bench_metric(D, naux):
+    """The FP64 aux-metric routines on an SPD matrix S = I + G G^T / n (G seeded
+    normal) at n = naux: the Cholesky factor (_dpotrf) and the factor of the
+    inverse (cholesky_decompose_inverse); at n = naux/2: lapack_power
+    (eigen-decomposition, S^-1/2 with eigenvalues below 1e-10 dropped)."""
+    from rest_tensors_amd._lib import lib, check
+    out = {}
+    for n, legs in ((naux, ("dpotrf", "cholesky_decompose_inverse")),
+                    (naux // 2, ("lapack_power",))):
+        dG, dW = D.DeviceArray(n * n), D.DeviceArray(n * n)
+        dS = D.DeviceArray.from_host(np.eye(n).reshape(-1))
+        D.gen_normal(dG, n * n, seed=31)
+        D.dgemm("N", "T", n, n, n, 1.0 / n, dG, n, dG, n, 1.0, dS, n)
+        for leg in legs:
+            reps = 1 if leg == "lapack_power" else 3
+            D.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                if leg == "dpotrf":
+                    check(lib().rt_memcpy_d2d(dW.ptr, dS.ptr, 8 * n * n))
+                    assert D.dpotrf("L", n, dW, n) == 0
+                elif leg == "cholesky_decompose_inverse":
+                    D.cholesky_decompose_inverse("L", n, dS, n, dW, n)
+                else:
+                    D.matrix_power(n, dS, n, -0.5, 1e-10, dW, n)
+            D.synchronize()
+            out[leg] = {"n": n, "ms": round((time.perf_counter() - t0) / reps * 1e3, 2)}
+        del dG, dS, dW
+    log("[bench] metric: " + ", ".join(f"{k} n={v['n']} {v['ms']} ms" for k, v in out.items()))
     return out
 
 

@@ -33,6 +33,7 @@ enum ScratchSlot {
   SCRATCH_DV_Z,
   SCRATCH_DV_YPART,
   SCRATCH_DV_ZPART,
+  SCRATCH_DV_PROJ,
   SCRATCH_LOWP_A,
   SCRATCH_LOWP_B,
   SCRATCH_MLP_A,

@@ -264,33 +264,6 @@ int dv_scale_cols(double* X, const double* scale, int nk, int64_t n,
   return RT_OK;
 }
 
-// ---- sequential Gram-Schmidt -----------------------------------------------------
-// One workgroup per column k of T: for i in 0..ns-1 (in order):
-//   T[:,k] -= Q[:,i] (Q[:,i] . T[:,k])       (davidson.rs:235-243)
-__global__ __launch_bounds__(1024) void project_out_kernel(
-    const double* __restrict__ Q, int64_t ldq, int ns, double* __restrict__ T,
-    int64_t n) {
-  __shared__ double red[40];
-  double* t = T + (int64_t)blockIdx.x * n;
-  for (int i = 0; i < ns; ++i) {
-    const double* q = Q + (int64_t)i * ldq;
-    double acc = 0.0;
-    for (int64_t x = threadIdx.x; x < n; x += blockDim.x) acc += q[x] * t[x];
-    const double d = dv_block_sum(acc, red);
-    for (int64_t x = threadIdx.x; x < n; x += blockDim.x) t[x] -= q[x] * d;
-    __syncthreads();
-  }
-}
-
-int dv_project_out(const double* Q, int64_t ldq, int ns, double* T, int nt,
-                   int64_t n, hipStream_t s) {
-  if (ns <= 0 || nt <= 0) return RT_OK;
-  KernelTimer kt("dv_project", s);
-  hipLaunchKernelGGL(project_out_kernel, dim3(nt), dim3(1024), 0, s, Q, ldq, ns,
-                     T, n);
-  RTA_HIP(hipGetLastError());
-  return RT_OK;
-}
 
 // T[:, c] -= sum_s Q[:, s] G[s + ns*c]
 __global__ __launch_bounds__(256) void sub_combine_kernel(const double* __restrict__ Q,

@@ -332,6 +332,15 @@ static int nsplit_for(int64_t n) {
 
 // Device workspace for ns samples: xcol (ns ints), ek, e0 (ns), stats
 // (MLP_NSTAT ns), partials.  Lives in scratch slots (grow on demand).
+// t (n) -= Q (Q^T t), twice, Q the qc orthonormal columns of a recorded
+// subspace (the block projection of the Davidson solve)
+static int project_one(const double* Q, int qc, double* t, int64_t n, hipStream_t s) {
+  double* G = nullptr;
+  RTA_TRY(scratch_get(SCRATCH_DV_PROJ, sizeof(double) * (qc + dv_gram_scratch(n, qc, 1)),
+                      (void**)&G));
+  return dv_project_out_block(Q, n, qc, t, 1, n, G, G + qc, s);
+}
+
 struct MlpWork {
   int* xcol;
   double* ek;
@@ -417,7 +426,7 @@ static int mlp_loss_grad(const MlpNet* net, const double* R, const double* X,
   if (Qs)
     for (int j = 0; j < ns; ++j)
       if (Qs[j] && qc[j] > 0) {
-        RTA_TRY(dv_project_out(Qs[j], n, qc[j], T + (size_t)j * n, 1, n, s));
+        RTA_TRY(project_one(Qs[j], qc[j], T + (size_t)j * n, n, s));
         projected = true;
       }
   const int nsp = nsplit_for(n);
@@ -682,7 +691,7 @@ extern "C" int rt_mlp_targets_dev(const double* X, const int* root, int ns,
       RTA_HIP(hipMemcpyAsync(tau + (size_t)j * n, Xfinal + (size_t)root[j] * n,
                              sizeof(double) * n, hipMemcpyDeviceToDevice, s));
     for (int j = 0; j < ns; ++j)
-      if (Qs[j]) RTA_TRY(dv_project_out(Qs[j], n, qc[j], tau + (size_t)j * n, 1, n, s));
+      if (Qs[j]) RTA_TRY(project_one(Qs[j], qc[j], tau + (size_t)j * n, n, s));
     RTA_TRY(dv_sqnorms(tau, ns, n, sq, sq + ns, s));
   }
   RTA_HIP(hipMemcpy(hsq.data(), sq, sizeof(double) * ns, hipMemcpyDeviceToHost));
