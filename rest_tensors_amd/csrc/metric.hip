@@ -34,46 +34,48 @@ __device__ __forceinline__ void panel_row(const double* __restrict__ M, int64_t 
   for (int j = 0; j < CH_NB; ++j) out[j] = row[ld * (c0 + (j < nc ? j : nc - 1))];
 }
 
-// One workgroup of 64 threads factors W[k0:k0+kb, k0:k0+kb] (lower) in LDS.
-// info (device): first failing pivot (1-based) if still 0.
+// One workgroup of 64 threads factors W[k0:k0+kb, k0:k0+kb] (lower),
+// right-looking: thread t keeps row t in registers; at step j thread j takes
+// the pivot, every row below scales its entry j, the column is shared
+// through LDS and each row updates its entries j < c <= t.  The steps are
+// unrolled, so the row stays in registers.  info (device): first failing
+// pivot (1-based) if still 0.
 __global__ __launch_bounds__(CH_NB) void potf2_kernel(double* __restrict__ W, int64_t ld,
                                                       int64_t k0, int kb,
                                                       int* __restrict__ info) {
-  __shared__ double L[CH_NB][CH_NB + 1];
+  __shared__ double col[CH_NB];
+  __shared__ double piv;
   const int t = threadIdx.x;
+  double r[CH_NB];
   {
     double v[CH_NB];
     panel_row(W, ld, k0, kb, k0, kb, t, v);
 #pragma unroll
-    for (int j = 0; j < CH_NB; ++j) L[t][j] = (t < kb && j < kb) ? v[j] : (t == j ? 1.0 : 0.0);
+    for (int j = 0; j < CH_NB; ++j) r[j] = (t < kb && j < kb) ? v[j] : (t == j ? 1.0 : 0.0);
   }
-  for (int j = 0; j < kb; ++j) {
-    __syncthreads();
+#pragma unroll
+  for (int j = 0; j < CH_NB; ++j) {
     if (t == j) {
-      const double d = L[j][j];
-      if (!(d > 0.0) && *info == 0) *info = (int)(k0 + j + 1);
-      L[j][j] = sqrt(d);
+      const double d = r[j];
+      if (j < kb && !(d > 0.0) && *info == 0) *info = (int)(k0 + j + 1);
+      r[j] = sqrt(d);
+      piv = r[j];
     }
     __syncthreads();
-    if (t > j && t < kb) L[t][j] /= L[j][j];
+    if (t > j) r[j] /= piv;
+    col[t] = r[j];
     __syncthreads();
-    if (t > j && t < kb) {
-      // independent updates, 4 in flight (the LDS latency, not the FMAs, sets the pace)
-      const double ltj = L[t][j];
-      int c = j + 1;
-      for (; c + 3 <= t; c += 4) {
-        const double a0 = L[c][j], a1 = L[c + 1][j], a2 = L[c + 2][j], a3 = L[c + 3][j];
-        L[t][c] -= ltj * a0;
-        L[t][c + 1] -= ltj * a1;
-        L[t][c + 2] -= ltj * a2;
-        L[t][c + 3] -= ltj * a3;
-      }
-      for (; c <= t; ++c) L[t][c] -= ltj * L[c][j];
+    if (t > j) {
+#pragma unroll
+      for (int c = j + 1; c < CH_NB; ++c)
+        if (c <= t) r[c] -= r[j] * col[c];
     }
+    __syncthreads();  // col and piv are rewritten by the next step
   }
-  __syncthreads();
   if (t < kb)
-    for (int j = 0; j <= t; ++j) W[(k0 + t) + ld * (k0 + j)] = L[t][j];
+#pragma unroll
+    for (int j = 0; j < CH_NB; ++j)
+      if (j <= t && j < kb) W[(k0 + t) + ld * (k0 + j)] = r[j];
 }
 
 // Rows r in [r0, r1): W[r, k0:k0+kb] <- W[r, k0:k0+kb] L11^-T (L11 = the
@@ -115,49 +117,6 @@ __global__ __launch_bounds__(CH_NB) void trsm_rows_kernel(double* __restrict__ W
   for (int j = 0; j < kb; ++j) W[(rb + t) + ld * (k0 + j)] = xs[t][j];
 }
 
-// Columns c in [0, nc): X[k0:k0+kb, c] <- L11^-1 X[k0:k0+kb, c] (forward
-// substitution with the diagonal block of the factor Lf); 64 columns per
-// workgroup, one per thread.
-__global__ __launch_bounds__(CH_NB) void trsm_cols_kernel(const double* __restrict__ Lf,
-                                                          int64_t ldl, double* __restrict__ X,
-                                                          int64_t ldx, int64_t k0, int kb,
-                                                          int64_t nc) {
-  __shared__ double L[CH_NB][CH_NB + 1];
-  __shared__ double ys[CH_NB][CH_NB + 1];
-  const int t = threadIdx.x;
-  const int64_t cb = blockIdx.x * (int64_t)CH_NB;
-  {
-    double v[CH_NB];
-    panel_row(Lf, ldl, k0, kb, k0, kb, t, v);
-#pragma unroll
-    for (int j = 0; j < CH_NB; ++j) L[t][j] = (t < kb && j < kb) ? v[j] : (t == j ? 1.0 : 0.0);
-    // ys[c][j] = X[k0 + j, cb + c]: thread t reads row k0+t across the 64 columns
-    const int ncb = (int)((nc - cb < CH_NB) ? nc - cb : CH_NB);
-    panel_row(X, ldx, k0, kb, cb, ncb, t, v);
-#pragma unroll
-    for (int c = 0; c < CH_NB; ++c) ys[c][t] = (t < kb && c < ncb) ? v[c] : 0.0;
-  }
-  __syncthreads();
-  if (cb + t < nc) {
-    for (int j = 0; j < kb; ++j) {
-      double v0 = ys[t][j], v1 = 0.0, v2 = 0.0, v3 = 0.0;
-      int i = 0;
-      for (; i + 3 < j; i += 4) {
-        v0 -= L[j][i] * ys[t][i];
-        v1 -= L[j][i + 1] * ys[t][i + 1];
-        v2 -= L[j][i + 2] * ys[t][i + 2];
-        v3 -= L[j][i + 3] * ys[t][i + 3];
-      }
-      for (; i < j; ++i) v0 -= L[j][i] * ys[t][i];
-      ys[t][j] = ((v0 + v1) + (v2 + v3)) / L[j][j];
-    }
-  }
-  __syncthreads();
-#pragma unroll 8
-  for (int c = 0; c < CH_NB; ++c)
-    if (t < kb && cb + c < nc) X[(k0 + t) + ldx * (cb + c)] = ys[c][t];
-}
-
 // S (n x n, ld n) = the symmetric matrix whose uplo triangle is A's
 __global__ void mt_symmetrize_kernel(const double* __restrict__ A, int64_t lda, int64_t n,
                                      bool upper, double* __restrict__ S) {
@@ -187,12 +146,6 @@ __global__ void mt_store_tri_kernel(const double* __restrict__ W, int64_t n, boo
 __global__ void mt_fill_kernel(double* __restrict__ X, int64_t len, double v) {
   const int64_t e = blockIdx.x * (int64_t)256 + threadIdx.x;
   if (e < len) X[e] = v;
-}
-
-// rows k0..k0+kb of X (ld n), columns [k0, k0+kb): identity block
-__global__ void mt_eye_block_kernel(double* __restrict__ X, int64_t n, int64_t k0, int kb) {
-  const int t = threadIdx.x;
-  if (t < kb) X[(k0 + t) + n * (k0 + t)] = 1.0;
 }
 
 // W (ld n) <- lower Cholesky factor of W in its lower triangle (the strict
