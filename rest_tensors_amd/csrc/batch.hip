@@ -13,11 +13,12 @@
 // rows.  The batch keeps its own zero-padded layouts (nbp, noccp = nb, nocc
 // rounded up to 16): ri3ao as Ap [nbp, nbp, naux] (copied once at creation,
 // again by rt_batch_reload), C as Cp [nbp, nb], X^T as [nbp, naux, noccp]
-// (rows (nu, P) = nu + nbp P) and B as Bp [naux, noccp, nb] (rows (P, o) =
-// P + naux o).  Each stage is then one GEMM per molecule with the aux index
-// inside the row index:
+// (rows (nu, P) = nu + nbp P) and B as Bp [ldbp, nb] (rows (P, o) = P + naux o
+// for o < nocc, ldbp = naux nocc rounded up to 16, the rows past naux nocc
+// zero).  Each stage is then one GEMM per molecule with the aux index inside
+// the row index:
 //   4  X^T[(nu,P), o]  = sum_mu Ap[mu, (nu,P)] Cp[mu, o]     M = nbp naux
-//   5  Bp[(P,o), i]    = sum_nu X^T[nu, (P,o)] Cp[nu, i]      M = naux noccp
+//   5  Bp[(P,o), i]    = sum_nu X^T[nu, (P,o)] Cp[nu, i]      M = naux nocc
 //   6  K slots         = sum over (P,o) of Bp[(P,o), i] Bp[(P,o), j], the
 //                        depth (P,o) contiguous, split into 16-deep batches
 //                        that the partial slots sum in order.
@@ -55,6 +56,7 @@ struct MolDev {
   double* Ap;       // padded ri3ao [nbp, nbp, naux] (TN path only)
   int64_t nb, naux, nocc;
   int64_t nbp, noccp;
+  int64_t ldbp;     // rows of Bp: naux*nocc rounded up to 16 (zero rows at the end)
   int64_t nchunks;  // K partial slots
   int64_t bchunk;   // per slot: aux slices (generic path) or 16-deep batches (TN path)
 };
@@ -301,7 +303,7 @@ __global__ __launch_bounds__(256) void batch_bcopy_kernel(const MolDev* __restri
   const int64_t npt = (M.naux + 31) / 32;
   const int64_t i = loc / npt, P0 = (loc % npt) * 32;
   const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
-  const double* src = M.Bp + M.naux * M.noccp * i;        // (P, o) at P + naux o
+  const double* src = M.Bp + M.ldbp * i;                  // (P, o) at P + naux o
   for (int64_t o0 = 0; o0 < M.nocc; o0 += kBcopyMaxOcc) {
     const int64_t oc = (M.nocc - o0 < kBcopyMaxOcc) ? M.nocc - o0 : kBcopyMaxOcc;
     __syncthreads();
@@ -555,14 +557,14 @@ static void build_tn_stage(Batch* b, int s, const GroupCfg& c) {
       p.ldb = m.nbp;
       p.C = m.Bp;
       p.c_rs = 1;
-      p.c_cs = m.naux * m.noccp;
-      p.M = m.naux * m.noccp;
+      p.c_cs = m.ldbp;
+      p.M = m.naux * m.nocc;  // rows (P, o), o < nocc
       p.N = m.nb;
       p.K = m.nbp;
     } else {  // K slots: depth (P,o) of each row of Bp, 16-deep batches
       p.A = m.Bp;
       p.B = m.Bp;
-      p.lda = p.ldb = m.naux * m.noccp;
+      p.lda = p.ldb = m.ldbp;
       p.sA = p.sB = 16;
       p.C = m.part;
       p.c_rs = 1;
@@ -570,7 +572,7 @@ static void build_tn_stage(Batch* b, int s, const GroupCfg& c) {
       p.sC = m.nb * m.nb;
       p.M = p.N = m.nb;
       p.K = 16;
-      p.nbatch = m.naux * m.noccp / 16;
+      p.nbatch = m.ldbp / 16;
       p.bchunk = m.bchunk;
       p.nchunks = m.nchunks;
       p.reduce = 1;
@@ -792,10 +794,11 @@ extern "C" int rt_batch_create(const rt_mol_desc* mols, int nmol, void** handle)
     d.nocc = m.nocc;
     d.nbp = (m.nbas + 15) / 16 * 16;
     d.noccp = (m.nocc + 15) / 16 * 16;
+    d.ldbp = (m.naux * m.nocc + 15) / 16 * 16;
     // K partial slots: up to 8, every slot non-empty (the reduce sums them
     // all); the TN path slices the depth (P, o) in 16-deep batches, the
     // generic one the aux slices
-    const int64_t nbt = use_tn ? m.naux * d.noccp / 16 : m.naux;
+    const int64_t nbt = use_tn ? d.ldbp / 16 : m.naux;
     const int64_t want = std::max<int64_t>(1, std::min<int64_t>(nbt, 8));
     d.bchunk = (nbt + want - 1) / want;
     d.nchunks = (nbt + d.bchunk - 1) / d.bchunk;
@@ -805,13 +808,15 @@ extern "C" int rt_batch_create(const rt_mol_desc* mols, int nmol, void** handle)
     if (st == RT_OK) st = dalloc(b, sizeof(double) * m.naux, (void**)&d.gamma);
     if (st == RT_OK)
       st = dalloc(b, sizeof(double) * d.nchunks * m.nbas * m.nbas, (void**)&d.part);
-    if (st == RT_OK) st = dalloc(b, sizeof(double) * d.noccp * m.nbas * m.naux, (void**)&d.Bp);
+    const size_t bpbytes = sizeof(double) * d.ldbp * m.nbas;
+    if (st == RT_OK) st = dalloc(b, bpbytes, (void**)&d.Bp);
     if (st == RT_OK) st = dalloc(b, cbytes, (void**)&d.Cp);
     if (st == RT_OK && use_tn)
       st = dalloc(b, sizeof(double) * d.nbp * d.nbp * m.naux, (void**)&d.Ap);
-    // the padding of X^T and Cp is zero for the life of the handle (never written)
+    // the padding of X^T, Cp and Bp is zero for the life of the handle (never written)
     if (st == RT_OK && (hipMemset(d.X, 0, xbytes) != hipSuccess ||
-                        hipMemset(d.Cp, 0, cbytes) != hipSuccess))
+                        hipMemset(d.Cp, 0, cbytes) != hipSuccess ||
+                        hipMemset(d.Bp, 0, bpbytes) != hipSuccess))
       st = fail(RT_ERR_HIP, "batch: clearing the padded buffers failed");
     if (st != RT_OK) return fail_cleanup(st);
     b->mols.push_back(d);
