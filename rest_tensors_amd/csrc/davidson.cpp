@@ -328,15 +328,9 @@ extern "C" int rt_davidson_solve_dev(rt_matvec_fn a_x, void* ctx,
                                hipMemcpyHostToDevice, s));
         RTA_TRY(dv_precond_diag(R.p, hdiag_dev, d_shift, nt, n, T.p, s));
       }
-      // normalize, project out the subspace, renormalize with lindep drops
-      RTA_TRY(dv_sqnorms(T.p, nt, n, d_sq, scr.p, s));
-      RTA_HIP(hipMemcpyAsync(h_sq.data(), d_sq, sizeof(double) * nt,
-                             hipMemcpyDeviceToHost, s));
-      RTA_HIP(hipStreamSynchronize(s));
-      for (int j = 0; j < nt; ++j) h_scale[j] = 1.0 / std::sqrt(h_sq[j]);
-      RTA_HIP(hipMemcpyAsync(d_scale, h_scale.data(), sizeof(double) * nt,
-                             hipMemcpyHostToDevice, s));
-      RTA_TRY(dv_scale_cols(T.p, d_scale, nt, n, s));
+      // normalize (on the device), project out the subspace, renormalize
+      // with lindep drops
+      RTA_TRY(dv_normalize_cols(T.p, nt, n, d_sq, scr.p, s));
       RTA_TRY(dv_project_out_block(XS.p, n, space, T.p, nt, n, d_gram, scr.p, s));
       RTA_TRY(dv_sqnorms(T.p, nt, n, d_sq, scr.p, s));
       RTA_HIP(hipMemcpyAsync(h_sq.data(), d_sq, sizeof(double) * nt,

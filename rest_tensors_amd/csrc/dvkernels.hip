@@ -255,6 +255,24 @@ __global__ void scale_cols_kernel(double* __restrict__ X,
   if (x < n) X[x + k * n] *= scale[k];
 }
 
+// X[:, k] *= 1 / sqrt(sq[k])   (sq on the device)
+__global__ void rnorm_cols_kernel(double* __restrict__ X, const double* __restrict__ sq,
+                                  int64_t n) {
+  const int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int k = blockIdx.y;
+  if (x < n) X[x + k * n] *= 1.0 / sqrt(sq[k]);
+}
+
+int dv_normalize_cols(double* X, int nk, int64_t n, double* sq, double* scratch,
+                      hipStream_t s) {
+  if (nk <= 0) return RT_OK;
+  RTA_TRY(dv_sqnorms(X, nk, n, sq, scratch, s));
+  hipLaunchKernelGGL(rnorm_cols_kernel, dim3((unsigned)((n + 255) / 256), nk), dim3(256), 0, s,
+                     X, sq, n);
+  RTA_HIP(hipGetLastError());
+  return RT_OK;
+}
+
 int dv_scale_cols(double* X, const double* scale, int nk, int64_t n,
                   hipStream_t s) {
   if (nk <= 0) return RT_OK;

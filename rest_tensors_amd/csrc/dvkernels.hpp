@@ -58,6 +58,10 @@ int dv_precond_diag(const double* R, const double* hdiag, const double* shift,
                     int nk, int64_t n, double* T, hipStream_t s);
 int dv_scale_cols(double* X, const double* scale, int nk, int64_t n,
                   hipStream_t s);
+// X[:, k] /= |X[:, k]| for k < nk, the norms computed and applied on the
+// device (sq: nk doubles of device output; scratch as dv_sqnorms)
+int dv_normalize_cols(double* X, int nk, int64_t n, double* sq, double* scratch,
+                      hipStream_t s);
 // T -= Q (Q^T T), twice (ns + nt columns; G >= ns*nt doubles, scratch >=
 // dv_gram_scratch(n, ns, nt)); classical Gram-Schmidt with reorthogonalization
 int dv_project_out_block(const double* Q, int64_t ldq, int ns, double* T, int nt, int64_t n,
