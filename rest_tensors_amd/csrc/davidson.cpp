@@ -19,7 +19,8 @@
 // (classical Gram-Schmidt with one reorthogonalization) instead of one
 // subspace vector at a time, and _qr runs its column-by-column loop with the
 // same drops on the small Gram matrix X^T X (host), then forms Q = X C on the
-// device, twice.  Both differ from the sequential order by rounding only and
+// device (a second time when the first pass cancelled heavily).  Both differ
+// from the sequential order by rounding only and
 // use the whole GPU instead of one workgroup per vector.  The subspace
 // eigenproblem is solved by Householder tridiagonalization and implicit QL
 // (the reference calls LAPACK's dspevx).

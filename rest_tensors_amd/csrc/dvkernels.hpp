@@ -67,7 +67,8 @@ int dv_normalize_cols(double* X, int nk, int64_t n, double* sq, double* scratch,
 int dv_project_out_block(const double* Q, int64_t ldq, int ns, double* T, int nt, int64_t n,
                          double* G, double* scratch, hipStream_t s);
 // _qr (modified Gram-Schmidt with lindep drops) as Gram-matrix Gram-Schmidt,
-// run twice: the accepted columns of X (ld n) go to Q (ld n), their count to
+// repeated once when a column kept less than 1 % of its squared norm (heavy
+// cancellation): the accepted columns of X (ld n) go to Q (ld n), their count to
 // *nout (host).  X is overwritten.  G, coef: device, >= nx^2 doubles;
 // scratch >= dv_gram_scratch(n, nx, nx); host: >= nx^2 doubles that stay
 // valid until the stream has passed this call.
