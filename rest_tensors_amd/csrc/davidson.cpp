@@ -142,11 +142,10 @@ extern "C" int rt_davidson_solve_dev(rt_matvec_fn a_x, void* ctx,
   const int cap = maxspace + 2 * nroots + 48;
   const int tmax = std::max(nroots, 40);
 
-  DevBuf XS, AX, X0, AX0, R, T, Sa, scr, small;
+  DevBuf XS, AX, X0, R, T, Sa, scr, small;
   RTA_TRY(XS.alloc((size_t)n * cap));
   RTA_TRY(AX.alloc((size_t)n * cap));
   RTA_TRY(X0.alloc((size_t)n * cap));
-  RTA_TRY(AX0.alloc((size_t)n * cap));
   RTA_TRY(R.alloc((size_t)n * tmax));
   RTA_TRY(T.alloc((size_t)n * tmax));
   if (prm->precond == RT_PRECOND_MLP && prm->aux_op) RTA_TRY(Sa.alloc((size_t)n * tmax));
@@ -244,9 +243,8 @@ extern "C" int rt_davidson_solve_dev(rt_matvec_fn a_x, void* ctx,
                            hipMemcpyHostToDevice, s));
     RTA_HIP(hipMemcpyAsync(d_e, e.data(), sizeof(double) * nkeep,
                            hipMemcpyHostToDevice, s));
-    RTA_TRY(dv_combine(XS.p, n, space, d_coef, space, nx0, n, X0.p, n, s));
-    RTA_TRY(dv_combine(AX.p, n, space, d_coef, space, nx0, n, AX0.p, n, s));
-    RTA_TRY(dv_residuals(AX0.p, X0.p, d_e, nroots_current, n, R.p, d_sq, scr.p, s));
+    RTA_TRY(dv_ritz_residuals(XS.p, AX.p, n, space, d_coef, space, nx0, d_e, n, X0.p, R.p,
+                              d_sq, scr.p, s));
     RTA_HIP(hipMemcpyAsync(h_sq.data(), d_sq, sizeof(double) * nroots_current,
                            hipMemcpyDeviceToHost, s));
     RTA_HIP(hipStreamSynchronize(s));

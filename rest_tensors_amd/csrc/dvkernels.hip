@@ -151,26 +151,7 @@ int dv_combine(const double* V, int64_t ldv, int ns, const double* coef,
   return RT_OK;
 }
 
-// ---- residuals R[:,k] = AX0[:,k] - e_k X0[:,k] and their norms ------------------
-__global__ __launch_bounds__(256) void residual_kernel(
-    const double* __restrict__ AX0, const double* __restrict__ X0,
-    const double* __restrict__ e, int64_t n, double* __restrict__ R,
-    double* __restrict__ sq_partial, int64_t slice) {
-  __shared__ double red[40];
-  const int k = blockIdx.y;
-  const int64_t x0 = (int64_t)blockIdx.x * slice;
-  const int64_t x1 = (x0 + slice < n) ? x0 + slice : n;
-  const double ek = e[k];
-  double acc = 0.0;
-  for (int64_t x = x0 + threadIdx.x; x < x1; x += 256) {
-    double r = AX0[x + k * n] - ek * X0[x + k * n];
-    R[x + k * n] = r;
-    acc += r * r;
-  }
-  acc = dv_block_sum(acc, red);
-  if (threadIdx.x == 0) sq_partial[k * gridDim.x + blockIdx.x] = acc;
-}
-
+// ---- Ritz vectors, residuals and column norms ------------------------------------
 // sum of squares of columns: out[k] = sum_s partial[k*ns + s]
 __global__ void colsum_kernel(const double* __restrict__ partial, int ns,
                               int nk, double* __restrict__ out) {
@@ -181,16 +162,49 @@ __global__ void colsum_kernel(const double* __restrict__ partial, int ns,
   out[k] = s;
 }
 
-int dv_residuals(const double* AX0, const double* X0, const double* e_dev,
-                 int nk, int64_t n, double* R, double* sq_out, double* scratch,
-                 hipStream_t s) {
+
+// Ritz vectors and residuals in one pass over the subspace blocks:
+//   X0[:, k] = XS C[:, k],  R[:, k] = AX C[:, k] - e_k X0[:, k]
+// (the sums over the subspace in the order of combine_kernel), with the
+// squared norm of each residual per row slice.
+__global__ __launch_bounds__(256) void ritz_residual_kernel(
+    const double* __restrict__ XS, const double* __restrict__ AX, int64_t ld, int ns,
+    const double* __restrict__ coef, int ldc, const double* __restrict__ e, int64_t n,
+    int64_t slice, double* __restrict__ X0, double* __restrict__ R,
+    double* __restrict__ sq_partial) {
+  __shared__ double red[40];
+  const int k = blockIdx.y;
+  const int64_t x0 = (int64_t)blockIdx.x * slice;
+  const int64_t x1 = (x0 + slice < n) ? x0 + slice : n;
+  const double ek = e[k];
+  const double* c = coef + (int64_t)ldc * k;
+  double acc = 0.0;
+  for (int64_t x = x0 + threadIdx.x; x < x1; x += 256) {
+    double xv = 0.0, av = 0.0;
+    for (int q = 0; q < ns; ++q) {
+      const double cq = c[q];
+      xv += XS[x + q * ld] * cq;
+      av += AX[x + q * ld] * cq;
+    }
+    X0[x + k * n] = xv;
+    const double r = av - ek * xv;
+    R[x + k * n] = r;
+    acc += r * r;
+  }
+  acc = dv_block_sum(acc, red);
+  if (threadIdx.x == 0) sq_partial[k * gridDim.x + blockIdx.x] = acc;
+}
+
+int dv_ritz_residuals(const double* XS, const double* AX, int64_t ld, int ns,
+                      const double* coef, int ldc, int nk, const double* e_dev, int64_t n,
+                      double* X0, double* R, double* sq_out, double* scratch, hipStream_t s) {
   if (nk <= 0) return RT_OK;
-  const int ns = gram_nsplit(n);
-  const int64_t slice = (n + ns - 1) / ns;
-  hipLaunchKernelGGL(residual_kernel, dim3(ns, nk), dim3(256), 0, s, AX0, X0,
-                     e_dev, n, R, scratch, slice);
-  hipLaunchKernelGGL(colsum_kernel, dim3(1), dim3(64), 0, s, scratch, ns, nk,
-                     sq_out);
+  // about one row per thread: 4 slices per Gram slice (scratch >= 4 nsplit nk)
+  const int nsl = 4 * gram_nsplit(n);
+  const int64_t slice = (n + nsl - 1) / nsl;
+  hipLaunchKernelGGL(ritz_residual_kernel, dim3(nsl, nk), dim3(256), 0, s, XS, AX, ld, ns, coef,
+                     ldc, e_dev, n, slice, X0, R, scratch);
+  hipLaunchKernelGGL(colsum_kernel, dim3(1), dim3(64), 0, s, scratch, nsl, nk, sq_out);
   RTA_HIP(hipGetLastError());
   return RT_OK;
 }

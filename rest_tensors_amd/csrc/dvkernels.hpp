@@ -48,10 +48,12 @@ int64_t dv_gram_scratch(int64_t n, int nu, int nw);
 int dv_combine(const double* V, int64_t ldv, int ns, const double* coef,
                int ldc, int nc, int64_t n, double* Y, int64_t ldy,
                hipStream_t s);
-// R[:,k] = AX0[:,k] - e[k] X0[:,k]; sq_out[k] = |R[:,k]|^2 (device)
-int dv_residuals(const double* AX0, const double* X0, const double* e_dev,
-                 int nk, int64_t n, double* R, double* sq_out, double* scratch,
-                 hipStream_t s);
+// X0[:,k] = XS C[:,k]; R[:,k] = AX C[:,k] - e[k] X0[:,k]; sq_out[k] = |R[:,k]|^2
+// (device), in one pass over the subspace blocks; scratch >= 4 nk doubles per
+// 1024-row slice (dv_gram_scratch(n, 4, nk))
+int dv_ritz_residuals(const double* XS, const double* AX, int64_t ld, int ns,
+                      const double* coef, int ldc, int nk, const double* e_dev, int64_t n,
+                      double* X0, double* R, double* sq_out, double* scratch, hipStream_t s);
 int dv_sqnorms(const double* X, int nk, int64_t n, double* sq_out,
                double* scratch, hipStream_t s);
 int dv_precond_diag(const double* R, const double* hdiag, const double* shift,
