@@ -406,6 +406,10 @@ struct Batch {
   int64_t* d_off_gquad = nullptr;  // (molecule, kGammaSlices aux slices), padded gamma
   int64_t n_gquad = 0;
   double tune_ms[4] = {0, 0, 0, 0};
+  // the RI-J branch (density, gamma, J) runs on a second stream beside the
+  // ao2mo/K branch; events fork it from and join it to the caller's stream
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
 };
@@ -698,24 +702,41 @@ static int launch_grouped(const GroupedStage& S, hipStream_t s) {
   return RT_OK;
 }
 
-static int run_stages(Batch* b, hipStream_t s) {
-  RTA_TRY(launch_grouped(b->st[0], s));  // density
+// RI-J (gamma -> J, HBM-bound) runs on the side stream beside the MFMA-bound
+// part of the ao2mo/K chain (stage 2, the B copy, the K slots): it is forked
+// after ao2mo stage 1, which streams the padded ri3ao itself, and joined
+// before the energies.  Under stream capture the fork and join become graph
+// edges.
+static int run_gamma_j(Batch* b, hipStream_t sj) {
   if (b->use_tn && b->n_gquad > 0)
     hipLaunchKernelGGL(batch_gamma_x2_kernel<kGammaSlices>, dim3((unsigned)b->n_gquad),
-                       dim3(256), 0, s, b->d_mols, b->d_off_gquad, b->nmol);
+                       dim3(256), 0, sj, b->d_mols, b->d_off_gquad, b->nmol);
   else if (!b->use_tn && b->n_gpair > 0)
-    hipLaunchKernelGGL(batch_gamma_kernel, dim3((unsigned)b->n_gpair), dim3(256), 0, s,
+    hipLaunchKernelGGL(batch_gamma_kernel, dim3((unsigned)b->n_gpair), dim3(256), 0, sj,
                        b->d_mols, b->d_off_gpair, b->nmol);
   if (b->n_j > 0)
-    hipLaunchKernelGGL(batch_j_kernel, dim3((unsigned)b->n_j), dim3(256), 0, s, b->d_mols,
+    hipLaunchKernelGGL(batch_j_kernel, dim3((unsigned)b->n_j), dim3(256), 0, sj, b->d_mols,
                        b->d_off_j, b->nmol);
   RTA_HIP(hipGetLastError());
+  return RT_OK;
+}
+
+static int run_stages(Batch* b, hipStream_t s) {
+  RTA_TRY(launch_grouped(b->st[0], s));  // density
+  auto fork = [&]() -> int {
+    RTA_HIP(hipEventRecord(b->ev_fork, s));
+    RTA_HIP(hipStreamWaitEvent(b->side, b->ev_fork, 0));
+    RTA_TRY(run_gamma_j(b, b->side));
+    RTA_HIP(hipEventRecord(b->ev_join, b->side));
+    return RT_OK;
+  };
   if (b->use_tn) {
     if (b->n_cpad > 0)
       hipLaunchKernelGGL(batch_cpad_kernel, dim3((unsigned)b->n_cpad), dim3(256), 0, s,
                          b->d_mols, b->d_off_cpad, b->nmol);
     RTA_HIP(hipGetLastError());
     RTA_TRY(launch_tn(b->tn[1], s, true));  // ao2mo stage 1 -> X^T
+    RTA_TRY(fork());
     RTA_TRY(launch_tn(b->tn[2], s, false));  // ao2mo stage 2 -> Bp
     if (b->n_bcopy > 0)
       hipLaunchKernelGGL(batch_bcopy_kernel, dim3((unsigned)b->n_bcopy), dim3(256), 0, s,
@@ -724,12 +745,14 @@ static int run_stages(Batch* b, hipStream_t s) {
     RTA_TRY(launch_tn(b->tn[3], s, false));  // K partials from Bp
   } else {
     RTA_TRY(launch_grouped(b->st[1], s));  // ao2mo stage 1
+    RTA_TRY(fork());
     RTA_TRY(launch_grouped(b->st[2], s));  // ao2mo stage 2
     RTA_TRY(launch_grouped(b->st[3], s));  // K partials
   }
   if (b->n_kred > 0)
     hipLaunchKernelGGL(batch_kreduce_kernel, dim3((unsigned)b->n_kred), dim3(256), 0, s,
                        b->d_mols, b->d_off_kred, b->nmol);
+  RTA_HIP(hipStreamWaitEvent(s, b->ev_join, 0));  // gamma and J are done
   hipLaunchKernelGGL(batch_energy_kernel, dim3(b->nmol), dim3(256), 0, s, b->d_mols,
                      b->energies);
   RTA_HIP(hipGetLastError());
@@ -767,9 +790,16 @@ extern "C" int rt_batch_create(const rt_mol_desc* mols, int nmol, void** handle)
   b->nmol = nmol;
   auto fail_cleanup = [&](int st) {
     for (void* p : b->owned) (void)hipFree(p);
+    if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);
+    if (b->ev_join) (void)hipEventDestroy(b->ev_join);
+    if (b->side) (void)hipStreamDestroy(b->side);
     delete b;
     return st;
   };
+  if (hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&b->ev_join, hipEventDisableTiming) != hipSuccess)
+    return fail_cleanup(fail(RT_ERR_HIP, "batch: stream or event creation failed"));
   // RTA_BATCH_GROUPED=1 keeps stages 4-6 on the generic grouped engine
   const char* forced = getenv("RTA_BATCH_GROUPED");
   const bool use_tn = !(forced && atoi(forced) == 1);
@@ -1074,7 +1104,11 @@ extern "C" int rt_batch_destroy(void* h) {
   Batch* b = (Batch*)h;
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
+  (void)hipDeviceSynchronize();  // the side stream's work is done before its buffers go
   for (void* p : b->owned) (void)hipFree(p);
+  if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);
+  if (b->ev_join) (void)hipEventDestroy(b->ev_join);
+  if (b->side) (void)hipStreamDestroy(b->side);
   delete b;
   return RT_OK;
 }
