@@ -80,6 +80,21 @@ def test_cholesky_decompose_inverse(D, n, uplo):
     assert np.array_equal(out, np.tril(out) if uplo == "L" else np.triu(out))
 
 
+@pytest.mark.parametrize("n", [700, 1025])
+def test_cholesky_decompose_inverse_ill_conditioned(D, n):
+    """Eigenvalues over 5 decades (cond 1e5): the recursive triangular inverse
+    stays within the conditioning of LAPACK's result; n = 1025 leaves a
+    one-row block for the top-level join."""
+    S = M.spd(n, seed=40 + n, lo=1e-5, hi=1.0)
+    dA, dO = D.DeviceArray.from_host(S), D.DeviceArray(n * n)
+    D.cholesky_decompose_inverse("L", n, dA, n, dO, n)
+    out = dO.to_host((n, n))
+    ref = M.cholesky_decompose_inverse(S, "L")
+    assert relf(out, ref) < 1e-9
+    # M M^T = S^-1: M^T S M = I to the conditioning
+    assert np.abs(out.T @ S @ out - np.eye(n)).max() < 1e-9
+
+
 def test_cholesky_decompose_inverse_not_pd(D):
     from rest_tensors_amd._lib import RtError
     S = np.diag([1.0, -1.0, 2.0])
