@@ -610,35 +610,41 @@ extern "C" int rt_mlp_train_dev(void* net_, const double* R, const double* X,
   std::vector<const double*> Qs_all, Qs(batch);
   std::vector<int> qc_all, qc(batch);
   mlp_subspaces(Q, q_off, q_cnt, ns, Qs_all, qc_all);
-  double *Rb, *Xb, *Ab = nullptr, *Tb, *taub, *g;
-  RTA_HIP(hipMalloc(&Rb, sizeof(double) * n * batch));
-  RTA_HIP(hipMalloc(&Xb, sizeof(double) * n * batch));
-  if (A) RTA_HIP(hipMalloc(&Ab, sizeof(double) * n * batch));
-  RTA_HIP(hipMalloc(&Tb, sizeof(double) * n * batch));
-  RTA_HIP(hipMalloc(&taub, sizeof(double) * n * batch));
-  RTA_HIP(hipMalloc(&g, sizeof(double) * MLP_NPARAM));
+  double *Rb = nullptr, *Xb = nullptr, *Ab = nullptr, *Tb = nullptr, *taub = nullptr,
+         *g = nullptr;
+  auto dmalloc = [](double** p, size_t cnt) {
+    if (hipMalloc(p, sizeof(double) * cnt) == hipSuccess) return true;
+    (void)hipGetLastError();
+    *p = nullptr;
+    return false;
+  };
+  const size_t nb = (size_t)n * batch;
+  int st = RT_OK;
+  if (!dmalloc(&Rb, nb) || !dmalloc(&Xb, nb) || (A && !dmalloc(&Ab, nb)) || !dmalloc(&Tb, nb) ||
+      !dmalloc(&taub, nb) || !dmalloc(&g, MLP_NPARAM))
+    st = fail(RT_ERR_NOMEM, "MLP training: device allocation failed");
   std::vector<double> ekb(batch), e0b(batch), wb(batch);
   const uint64_t key = rng_key(seed);
-  int st = RT_OK;
   const float b1 = 0.9f, b2 = 0.999f;
   for (int it = 0; it < steps && st == RT_OK; ++it) {
-    for (int j = 0; j < batch; ++j) {
+    for (int j = 0; j < batch && st == RT_OK; ++j) {
       const int q = (int)(splitmix64(key + (uint64_t)it * batch + j) % (uint64_t)ns);
-      hipMemcpyAsync(Rb + n * j, R + n * q, sizeof(double) * n,
-                     hipMemcpyDeviceToDevice, s);
-      hipMemcpyAsync(Xb + n * j, X + n * q, sizeof(double) * n,
-                     hipMemcpyDeviceToDevice, s);
-      if (A)
-        hipMemcpyAsync(Ab + n * j, A + n * q, sizeof(double) * n,
-                       hipMemcpyDeviceToDevice, s);
-      hipMemcpyAsync(taub + n * j, tau + n * q, sizeof(double) * n,
-                     hipMemcpyDeviceToDevice, s);
+      const size_t bytes = sizeof(double) * n;
+      hipError_t e = hipMemcpyAsync(Rb + n * j, R + n * q, bytes, hipMemcpyDeviceToDevice, s);
+      if (e == hipSuccess)
+        e = hipMemcpyAsync(Xb + n * j, X + n * q, bytes, hipMemcpyDeviceToDevice, s);
+      if (e == hipSuccess && A)
+        e = hipMemcpyAsync(Ab + n * j, A + n * q, bytes, hipMemcpyDeviceToDevice, s);
+      if (e == hipSuccess)
+        e = hipMemcpyAsync(taub + n * j, tau + n * q, bytes, hipMemcpyDeviceToDevice, s);
+      if (e != hipSuccess) st = fail(RT_ERR_HIP, "MLP training: batch gather failed");
       ekb[j] = ek[q];
       e0b[j] = e0[q];
       wb[j] = wts[q];
       Qs[j] = Qs_all[q];
       qc[j] = qc_all[q];
     }
+    if (st != RT_OK) break;
     double L = 0.0;
     st = mlp_loss_grad(net, Rb, Xb, Ab, hdiag, ekb.data(), e0b.data(), taub, wb.data(),
                        n, batch, Qs.data(), qc.data(), &L, g, Tb, s);
