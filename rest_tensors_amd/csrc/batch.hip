@@ -3,7 +3,9 @@
 //
 // Every stage of the FP64 pipeline is ONE launch covering all molecules:
 //   1 density      D_m = C_occ C_occ^T          grouped GEMM (persistent)
-//   2 gamma        gamma_m,P = <A_m,P, D_m>      grouped, one block per (m, 4 slices)
+//   2 gamma        gamma_m,P = <A_m,P, D_m>      TN path: sum_o B_m[o, o, P] after
+//                                                stage 5; generic path: one block
+//                                                per (m, 2 slices) over ri3ao
 //   3 J            J_m = sum_P A_m,P gamma_m,P   grouped, one lane per packed (i, j)
 //   4 ao2mo s1     X^T = A_P^T C_occ             grouped TN GEMM (direct-to-LDS)
 //   5 ao2mo s2     B_P = (X^T)^T C               grouped TN GEMM
@@ -117,80 +119,23 @@ __global__ __launch_bounds__(256) void batch_gamma_kernel(
 // The same sums from the padded copy Ap (leading dimension nbp, a multiple of
 // 16, so every column starts 16-byte aligned), one block per (molecule, SP
 // aux slices): a lane takes the pair of rows (2t, 2t+1) of a column with one
-// 16-byte load per slice (weights 2 above the diagonal, 1 on it, 0 for the
-// row below it, which is the zero pad or the next row and never past the
-// column); wave w walks columns j = w, w+4, ... in chunks of 64 pairs, the
-// next chunk's loads issued before the current one is summed.  D is the
-// unpadded [nb, nb] (its rows read one by one, once for all SP slices).
-template <int SP>
-__global__ __launch_bounds__(256) void batch_gamma_x2_kernel(
+// TN path: with D = C_occ C_occ^T, gamma_P = <A_P, D> = sum_o B[o, o, P]
+// (B_P = C_occ^T A_P C), so gamma is read off the occupied diagonal of Bp
+// (row (P, o) = P + naux o, column i: B[o, i, P]) after ao2mo stage 2 instead
+// of from another pass over ri3ao.  One lane per (molecule, P), the o-sum in
+// order.
+__global__ __launch_bounds__(256) void batch_gamma_bdiag_kernel(
     const MolDev* __restrict__ mols, const int64_t* __restrict__ offs, int nmol) {
-  __shared__ double red[SP][4];
   const int m = find_mol(offs, nmol, blockIdx.x);
   const MolDev M = mols[m];
-  const int64_t P0 = SP * (blockIdx.x - offs[m]);
-  const int64_t nb = M.nb, ldh = M.nbp / 2, naux = M.naux;
-  const double2* Ab[SP];
-#pragma unroll
-  for (int q = 0; q < SP; ++q)
-    Ab[q] = reinterpret_cast<const double2*>(M.Ap +
-                                             (P0 + q < naux ? P0 + q : P0) * M.nbp * M.nbp);
-  const double* D = M.D;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  auto load = [&](int64_t j, int64_t u, double2* a, double& d0, double& d1) {
-    const int64_t r = 2 * (u + lane);
-    if (r <= j) {
-#pragma unroll
-      for (int q = 0; q < SP; ++q) a[q] = Ab[q][j * ldh + u + lane];
-      const double* dc = D + j * nb + r;
-      d0 = (r < j ? 2.0 : 1.0) * dc[0];
-      d1 = r + 1 < j ? 2.0 * dc[1] : (r + 1 == j ? dc[1] : 0.0);
-    }
-  };
-  double acc[SP];
-  double2 a[SP];
-#pragma unroll
-  for (int q = 0; q < SP; ++q) {
-    acc[q] = 0.0;
-    a[q] = make_double2(0.0, 0.0);
-  }
-  double d0 = 0.0, d1 = 0.0;
-  int64_t j = wave, u = 0;
-  if (j < nb) load(j, u, a, d0, d1);
-  while (j < nb) {
-    int64_t jn = j, un = u + 64;
-    if (2 * un > jn) {
-      un = 0;
-      jn += 4;
-    }
-    double2 an[SP];
-#pragma unroll
-    for (int q = 0; q < SP; ++q) an[q] = make_double2(0.0, 0.0);
-    double e0 = 0.0, e1 = 0.0;
-    if (jn < nb) load(jn, un, an, e0, e1);
-#pragma unroll
-    for (int q = 0; q < SP; ++q) {
-      acc[q] += a[q].x * d0 + a[q].y * d1;
-      a[q] = an[q];
-    }
-    j = jn;
-    u = un;
-    d0 = e0;
-    d1 = e1;
-  }
-#pragma unroll
-  for (int q = 0; q < SP; ++q) acc[q] = b_wave_sum(acc[q]);
-  if (lane == 0) {
-#pragma unroll
-    for (int q = 0; q < SP; ++q) red[q][wave] = acc[q];
-  }
-  __syncthreads();
-  if (threadIdx.x < SP && P0 + threadIdx.x < naux) {
-    const int q = threadIdx.x;
-    M.gamma[P0 + q] = (red[q][0] + red[q][1]) + (red[q][2] + red[q][3]);
-  }
+  const int64_t P = (blockIdx.x - offs[m]) * 256 + threadIdx.x;
+  if (P >= M.naux) return;
+  const double* p = M.Bp + P;
+  const int64_t step = M.naux + M.ldbp;  // (P, o) -> (P, o + 1), column o -> o + 1
+  double acc = 0.0;
+  for (int64_t o = 0; o < M.nocc; ++o) acc += p[o * step];
+  M.gamma[P] = acc;
 }
-constexpr int kGammaSlices = 4;  // aux slices per block of batch_gamma_x2_kernel
 
 // one block per (molecule, 256 consecutive pairs of the packed upper
 // triangle, q = j(j+1)/2 + i): every lane owns one pair (i, j) and sums all
@@ -403,8 +348,8 @@ struct Batch {
   int64_t n_bcopy = 0;
   int64_t* d_off_gpair = nullptr;  // (molecule, pair of aux slices) offsets for gamma
   int64_t n_gpair = 0;
-  int64_t* d_off_gquad = nullptr;  // (molecule, kGammaSlices aux slices), padded gamma
-  int64_t n_gquad = 0;
+  int64_t* d_off_gdiag = nullptr;  // (molecule, 256 aux slices): gamma from Bp
+  int64_t n_gdiag = 0;
   double tune_ms[4] = {0, 0, 0, 0};
   // the RI-J branch (density, gamma, J) runs on a second stream beside the
   // ao2mo/K branch; events fork it from and join it to the caller's stream
@@ -702,15 +647,16 @@ static int launch_grouped(const GroupedStage& S, hipStream_t s) {
   return RT_OK;
 }
 
-// RI-J (gamma -> J, HBM-bound) runs on the side stream beside the MFMA-bound
-// part of the ao2mo/K chain (stage 2, the B copy, the K slots): it is forked
-// after ao2mo stage 1, which streams the padded ri3ao itself, and joined
-// before the energies.  Under stream capture the fork and join become graph
-// edges.
+// RI-J (gamma -> J; J is one HBM-bound pass over the upper triangles of
+// ri3ao) runs on the side stream beside the MFMA-bound rest of the ao2mo/K
+// chain and is joined before the energies.  TN path: forked after ao2mo
+// stage 2, gamma is read off Bp's occupied diagonal and J overlaps the B copy
+// and the K slots.  Generic path: forked after stage 1, gamma is its own pass
+// over ri3ao.  Under stream capture the fork and join become graph edges.
 static int run_gamma_j(Batch* b, hipStream_t sj) {
-  if (b->use_tn && b->n_gquad > 0)
-    hipLaunchKernelGGL(batch_gamma_x2_kernel<kGammaSlices>, dim3((unsigned)b->n_gquad),
-                       dim3(256), 0, sj, b->d_mols, b->d_off_gquad, b->nmol);
+  if (b->use_tn && b->n_gdiag > 0)
+    hipLaunchKernelGGL(batch_gamma_bdiag_kernel, dim3((unsigned)b->n_gdiag), dim3(256), 0, sj,
+                       b->d_mols, b->d_off_gdiag, b->nmol);
   else if (!b->use_tn && b->n_gpair > 0)
     hipLaunchKernelGGL(batch_gamma_kernel, dim3((unsigned)b->n_gpair), dim3(256), 0, sj,
                        b->d_mols, b->d_off_gpair, b->nmol);
@@ -735,9 +681,9 @@ static int run_stages(Batch* b, hipStream_t s) {
       hipLaunchKernelGGL(batch_cpad_kernel, dim3((unsigned)b->n_cpad), dim3(256), 0, s,
                          b->d_mols, b->d_off_cpad, b->nmol);
     RTA_HIP(hipGetLastError());
-    RTA_TRY(launch_tn(b->tn[1], s, true));  // ao2mo stage 1 -> X^T
-    RTA_TRY(fork());
+    RTA_TRY(launch_tn(b->tn[1], s, true));   // ao2mo stage 1 -> X^T
     RTA_TRY(launch_tn(b->tn[2], s, false));  // ao2mo stage 2 -> Bp
+    RTA_TRY(fork());
     if (b->n_bcopy > 0)
       hipLaunchKernelGGL(batch_bcopy_kernel, dim3((unsigned)b->n_bcopy), dim3(256), 0, s,
                          b->d_mols, b->d_off_bcopy, b->nmol);
@@ -804,9 +750,9 @@ extern "C" int rt_batch_create(const rt_mol_desc* mols, int nmol, void** handle)
   const char* forced = getenv("RTA_BATCH_GROUPED");
   const bool use_tn = !(forced && atoi(forced) == 1);
   b->use_tn = use_tn;
-  int64_t ng = 0, nj = 0, nk = 0, nc = 0, nbc = 0, ngp = 0, ngq = 0;
+  int64_t ng = 0, nj = 0, nk = 0, nc = 0, nbc = 0, ngp = 0, ngd = 0;
   std::vector<int64_t> og(1, 0), oj(1, 0), ok(1, 0), oc(1, 0), obc(1, 0), ogp(1, 0),
-      ogq(1, 0);
+      ogd(1, 0);
   for (int i = 0; i < nmol; ++i) {
     const rt_mol_desc& m = mols[i];
     if (!(m.nbas > 0 && m.naux > 0 && m.nocc > 0 && m.nocc <= m.nbas && m.ri3ao && m.C &&
@@ -859,8 +805,8 @@ extern "C" int rt_batch_create(const rt_mol_desc* mols, int nmol, void** handle)
     obc.push_back(nbc);
     ngp += (m.naux + 1) / 2;
     ogp.push_back(ngp);
-    ngq += (m.naux + kGammaSlices - 1) / kGammaSlices;
-    ogq.push_back(ngq);
+    ngd += (m.naux + 255) / 256;
+    ogd.push_back(ngd);
     og.push_back(ng);
     oj.push_back(nj);
     ok.push_back(nk);
@@ -872,7 +818,7 @@ extern "C" int rt_batch_create(const rt_mol_desc* mols, int nmol, void** handle)
   b->n_cpad = nc;
   b->n_bcopy = nbc;
   b->n_gpair = ngp;
-  b->n_gquad = ngq;
+  b->n_gdiag = ngd;
 
   int st = dalloc(b, sizeof(MolDev) * nmol, (void**)&b->d_mols);
   if (st == RT_OK) st = dalloc(b, sizeof(int64_t) * og.size(), (void**)&b->d_off_gamma);
@@ -881,7 +827,7 @@ extern "C" int rt_batch_create(const rt_mol_desc* mols, int nmol, void** handle)
   if (st == RT_OK) st = dalloc(b, sizeof(int64_t) * oc.size(), (void**)&b->d_off_cpad);
   if (st == RT_OK) st = dalloc(b, sizeof(int64_t) * obc.size(), (void**)&b->d_off_bcopy);
   if (st == RT_OK) st = dalloc(b, sizeof(int64_t) * ogp.size(), (void**)&b->d_off_gpair);
-  if (st == RT_OK) st = dalloc(b, sizeof(int64_t) * ogq.size(), (void**)&b->d_off_gquad);
+  if (st == RT_OK) st = dalloc(b, sizeof(int64_t) * ogd.size(), (void**)&b->d_off_gdiag);
   if (st == RT_OK) st = dalloc(b, sizeof(double) * 2 * nmol, (void**)&b->energies);
   if (st != RT_OK) return fail_cleanup(st);
   if (hipMemcpy(b->d_mols, b->mols.data(), sizeof(MolDev) * nmol, hipMemcpyHostToDevice) !=
@@ -898,7 +844,7 @@ extern "C" int rt_batch_create(const rt_mol_desc* mols, int nmol, void** handle)
                 hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(b->d_off_gpair, ogp.data(), sizeof(int64_t) * ogp.size(),
                 hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(b->d_off_gquad, ogq.data(), sizeof(int64_t) * ogq.size(),
+      hipMemcpy(b->d_off_gdiag, ogd.data(), sizeof(int64_t) * ogd.size(),
                 hipMemcpyHostToDevice) != hipSuccess)
     return fail_cleanup(fail(RT_ERR_HIP, "batch: descriptor upload failed"));
   for (int s = 0; s < 4; ++s) {
