@@ -673,6 +673,10 @@ static int run_stages(Batch* b, hipStream_t s) {
     RTA_HIP(hipEventRecord(b->ev_fork, s));
     RTA_HIP(hipStreamWaitEvent(b->side, b->ev_fork, 0));
     RTA_TRY(run_gamma_j(b, b->side));
+    if (b->use_tn && b->n_bcopy > 0)  // Bp -> caller's B, beside the K slots (both read Bp)
+      hipLaunchKernelGGL(batch_bcopy_kernel, dim3((unsigned)b->n_bcopy), dim3(256), 0, b->side,
+                         b->d_mols, b->d_off_bcopy, b->nmol);
+    RTA_HIP(hipGetLastError());
     RTA_HIP(hipEventRecord(b->ev_join, b->side));
     return RT_OK;
   };
@@ -684,10 +688,6 @@ static int run_stages(Batch* b, hipStream_t s) {
     RTA_TRY(launch_tn(b->tn[1], s, true));   // ao2mo stage 1 -> X^T
     RTA_TRY(launch_tn(b->tn[2], s, false));  // ao2mo stage 2 -> Bp
     RTA_TRY(fork());
-    if (b->n_bcopy > 0)
-      hipLaunchKernelGGL(batch_bcopy_kernel, dim3((unsigned)b->n_bcopy), dim3(256), 0, s,
-                         b->d_mols, b->d_off_bcopy, b->nmol);
-    RTA_HIP(hipGetLastError());
     RTA_TRY(launch_tn(b->tn[3], s, false));  // K partials from Bp
   } else {
     RTA_TRY(launch_grouped(b->st[1], s));  // ao2mo stage 1
@@ -698,7 +698,7 @@ static int run_stages(Batch* b, hipStream_t s) {
   if (b->n_kred > 0)
     hipLaunchKernelGGL(batch_kreduce_kernel, dim3((unsigned)b->n_kred), dim3(256), 0, s,
                        b->d_mols, b->d_off_kred, b->nmol);
-  RTA_HIP(hipStreamWaitEvent(s, b->ev_join, 0));  // gamma and J are done
+  RTA_HIP(hipStreamWaitEvent(s, b->ev_join, 0));  // gamma, J (and the B copy) are done
   hipLaunchKernelGGL(batch_energy_kernel, dim3(b->nmol), dim3(256), 0, s, b->d_mols,
                      b->energies);
   RTA_HIP(hipGetLastError());
