@@ -110,6 +110,26 @@ struct DevBuf {
   }
 };
 
+// Pinned host memory for the per-iteration transfers (subspace matrices,
+// coefficients, norms): pageable transfers go through a staging copy and
+// cost tens of microseconds each, several times per iteration.
+struct HostBuf {
+  double* p = nullptr;
+  ~HostBuf() {
+    if (p) (void)hipHostFree(p);
+  }
+  int alloc(size_t count) {
+    if (hipHostMalloc((void**)&p, sizeof(double) * std::max<size_t>(count, 1),
+                      hipHostMallocDefault) != hipSuccess) {
+      (void)hipGetLastError();
+      p = nullptr;
+      return fail(RT_ERR_NOMEM, "davidson: pinned host allocation of %zu doubles failed",
+                  count);
+    }
+    return RT_OK;
+  }
+};
+
 static double now_s() {
   using namespace std::chrono;
   return duration<double>(steady_clock::now().time_since_epoch()).count();
@@ -175,8 +195,19 @@ extern "C" int rt_davidson_solve_dev(rt_matvec_fn a_x, void* ctx,
   bool fresh = true;
   std::vector<double> heff, w, V, e, elast;
   std::vector<int> conv(nroots, 0);
-  std::vector<double> dx_norm, de, h_sq(cap), h_gram((size_t)cap * tmax),
-      h_coef((size_t)cap * cap), h_shift(cap), h_scale(cap), h_qr((size_t)tmax * tmax);
+  std::vector<double> dx_norm, de;
+  // pinned host area, laid out like the small device area: e, shift, scale,
+  // sq (cap each), coef (cap*cap), gram (cap*tmax), then the _qr Gram matrix
+  // and coefficients (2*tmax*tmax)
+  HostBuf hbuf;
+  RTA_TRY(hbuf.alloc(o_end + 2 * (size_t)tmax * tmax));
+  double* h_e = hbuf.p + o_e;
+  double* h_shift = hbuf.p + o_shift;
+  double* h_scale = hbuf.p + o_scale;
+  double* h_sq = hbuf.p + o_sq;
+  double* h_coef = hbuf.p + o_coef;
+  double* h_gram = hbuf.p + o_gram;
+  double* h_qr = hbuf.p + o_end;
   int nroots_current = 0;
   rt_davidson_stats st;
   memset(&st, 0, sizeof st);
@@ -186,13 +217,13 @@ extern "C" int rt_davidson_solve_dev(rt_matvec_fn a_x, void* ctx,
     int xtlen = 0;
     if (fresh) {
       space = 0;
-      RTA_TRY(dv_qr(X0.p, nx0, n, lindep, XS.p, &xtlen, d_gram, d_coef, scr.p, h_qr.data(), s));
+      RTA_TRY(dv_qr(X0.p, nx0, n, lindep, XS.p, &xtlen, d_gram, d_coef, scr.p, h_qr, s));
       if (xtlen == 0)
         return fail(RT_ERR_NUMERIC, "No more linear independent basis found");
       ++st.restarts;
     } else if (nt > 1) {
       RTA_TRY(dv_qr(T.p, nt, n, lindep, XS.p + (size_t)space * n, &xtlen, d_gram, d_coef,
-                    scr.p, h_qr.data(), s));
+                    scr.p, h_qr, s));
       xtlen = std::min(xtlen, 40);
     } else {
       RTA_HIP(hipMemcpyAsync(XS.p + (size_t)space * n, T.p, sizeof(double) * n,
@@ -210,7 +241,7 @@ extern "C" int rt_davidson_solve_dev(rt_matvec_fn a_x, void* ctx,
     // heff: H[i,j] = H[j,i] = ax_i . x_j for i <= j, j new (davidson.rs:356-387)
     RTA_TRY(dv_gram(AX.p, n, space, XS.p + (size_t)row0 * n, n, xtlen, n, d_gram,
                     space, scr.p, s));
-    RTA_HIP(hipMemcpyAsync(h_gram.data(), d_gram, sizeof(double) * space * xtlen,
+    RTA_HIP(hipMemcpyAsync(h_gram, d_gram, sizeof(double) * space * xtlen,
                            hipMemcpyDeviceToHost, s));
     RTA_HIP(hipStreamSynchronize(s));
     {
@@ -239,13 +270,14 @@ extern "C" int rt_davidson_solve_dev(rt_matvec_fn a_x, void* ctx,
     for (int c = 0; c < nx0; ++c)
       for (int i = 0; i < space; ++i)
         h_coef[i + (size_t)space * c] = V[i + (size_t)space * c];
-    RTA_HIP(hipMemcpyAsync(d_coef, h_coef.data(), sizeof(double) * space * nx0,
+    for (int k = 0; k < nkeep; ++k) h_e[k] = e[k];
+    RTA_HIP(hipMemcpyAsync(d_coef, h_coef, sizeof(double) * space * nx0,
                            hipMemcpyHostToDevice, s));
-    RTA_HIP(hipMemcpyAsync(d_e, e.data(), sizeof(double) * nkeep,
+    RTA_HIP(hipMemcpyAsync(d_e, h_e, sizeof(double) * nkeep,
                            hipMemcpyHostToDevice, s));
     RTA_TRY(dv_ritz_residuals(XS.p, AX.p, n, space, d_coef, space, nx0, d_e, n, X0.p, R.p,
                               d_sq, scr.p, s));
-    RTA_HIP(hipMemcpyAsync(h_sq.data(), d_sq, sizeof(double) * nroots_current,
+    RTA_HIP(hipMemcpyAsync(h_sq, d_sq, sizeof(double) * nroots_current,
                            hipMemcpyDeviceToHost, s));
     RTA_HIP(hipStreamSynchronize(s));
     dx_norm.assign(nroots_current, 0.0);
@@ -323,7 +355,7 @@ extern "C" int rt_davidson_solve_dev(rt_matvec_fn a_x, void* ctx,
       } else {
         for (int j = 0; j < nt; ++j)
           h_shift[j] = (prm->precond == RT_PRECOND_DIAG_ROOT) ? e[keep[j]] : e[0];
-        RTA_HIP(hipMemcpyAsync(d_shift, h_shift.data(), sizeof(double) * nt,
+        RTA_HIP(hipMemcpyAsync(d_shift, h_shift, sizeof(double) * nt,
                                hipMemcpyHostToDevice, s));
         RTA_TRY(dv_precond_diag(R.p, hdiag_dev, d_shift, nt, n, T.p, s));
       }
@@ -332,7 +364,7 @@ extern "C" int rt_davidson_solve_dev(rt_matvec_fn a_x, void* ctx,
       RTA_TRY(dv_normalize_cols(T.p, nt, n, d_sq, scr.p, s));
       RTA_TRY(dv_project_out_block(XS.p, n, space, T.p, nt, n, d_gram, scr.p, s));
       RTA_TRY(dv_sqnorms(T.p, nt, n, d_sq, scr.p, s));
-      RTA_HIP(hipMemcpyAsync(h_sq.data(), d_sq, sizeof(double) * nt,
+      RTA_HIP(hipMemcpyAsync(h_sq, d_sq, sizeof(double) * nt,
                              hipMemcpyDeviceToHost, s));
       RTA_HIP(hipStreamSynchronize(s));
       int m = 0;
@@ -347,7 +379,7 @@ extern "C" int rt_davidson_solve_dev(rt_matvec_fn a_x, void* ctx,
       }
       nt = m;
       if (nt > 0) {
-        RTA_HIP(hipMemcpyAsync(d_scale, h_scale.data(), sizeof(double) * nt,
+        RTA_HIP(hipMemcpyAsync(d_scale, h_scale, sizeof(double) * nt,
                                hipMemcpyHostToDevice, s));
         RTA_TRY(dv_scale_cols(T.p, d_scale, nt, n, s));
       }

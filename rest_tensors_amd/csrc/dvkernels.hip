@@ -328,7 +328,7 @@ int dv_project_out_block(const double* Q, int64_t ldq, int ns, double* T, int nt
 // coefficient columns c_j in order, kept as v / |v|_G when v^T G v > lindep.
 // The accepted columns go to C (k x nv, ld k); returns nv.  *min_frac: the
 // smallest share v^T G v / G_ii of its squared norm an accepted column kept.
-static int gs_gram_host(const std::vector<double>& G, int k, double lindep, double* C,
+static int gs_gram_host(const double* G, int k, double lindep, double* C,
                         double* min_frac) {
   std::vector<double> v(k), Gv(k);
   int nv = 0;
@@ -367,7 +367,8 @@ int dv_qr(double* X, int nx, int64_t n, double lindep, double* Q, int* nout, dou
   KernelTimer kt("dv_mgs", s);
   *nout = 0;
   int k = nx;
-  std::vector<double> Gh;
+  double* Gh = host;            // k x k Gram matrix
+  double* Ch = host + (size_t)nx * nx;  // k x nv coefficients
   // a second pass re-orthogonalizes only when the first cancelled heavily: a
   // column that kept at least 1 % of its squared norm loses orthogonality to
   // about eps / 0.01, which the Rayleigh-Ritz step does not notice
@@ -378,14 +379,13 @@ int dv_qr(double* X, int nx, int64_t n, double lindep, double* Q, int* nout, dou
     const double* src = pass == 0 ? X : Q;
     double* dst = pass == 0 ? Q : X;
     RTA_TRY(dv_gram(src, n, k, src, n, k, n, G, k, scratch, s));
-    Gh.assign((size_t)k * k, 0.0);
-    RTA_HIP(hipMemcpyAsync(Gh.data(), G, sizeof(double) * k * k, hipMemcpyDeviceToHost, s));
+    RTA_HIP(hipMemcpyAsync(Gh, G, sizeof(double) * k * k, hipMemcpyDeviceToHost, s));
     RTA_HIP(hipStreamSynchronize(s));
     // the synchronization above also retired the previous pass's upload and
     // combine, so host and coef are free again
-    const int nv = gs_gram_host(Gh, k, lindep, host, &min_frac);
+    const int nv = gs_gram_host(Gh, k, lindep, Ch, &min_frac);
     if (nv > 0) {
-      RTA_HIP(hipMemcpyAsync(coef, host, sizeof(double) * k * nv, hipMemcpyHostToDevice, s));
+      RTA_HIP(hipMemcpyAsync(coef, Ch, sizeof(double) * k * nv, hipMemcpyHostToDevice, s));
       RTA_TRY(dv_combine(src, n, k, coef, k, nv, n, dst, n, s));
       if (pass == 1)
         RTA_HIP(hipMemcpyAsync(Q, X, sizeof(double) * n * nv, hipMemcpyDeviceToDevice, s));

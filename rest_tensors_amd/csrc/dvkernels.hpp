@@ -72,8 +72,9 @@ int dv_project_out_block(const double* Q, int64_t ldq, int ns, double* T, int nt
 // repeated once when a column kept less than 1 % of its squared norm (heavy
 // cancellation): the accepted columns of X (ld n) go to Q (ld n), their count to
 // *nout (host).  X is overwritten.  G, coef: device, >= nx^2 doubles;
-// scratch >= dv_gram_scratch(n, nx, nx); host: >= nx^2 doubles that stay
-// valid until the stream has passed this call.
+// scratch >= dv_gram_scratch(n, nx, nx); host: >= 2 nx^2 doubles (the Gram
+// matrix and the coefficients; pinned memory keeps the two transfers short)
+// that stay valid until the stream has passed this call.
 int dv_qr(double* X, int nx, int64_t n, double lindep, double* Q, int* nout, double* G,
           double* coef, double* scratch, double* host, hipStream_t s);
 
