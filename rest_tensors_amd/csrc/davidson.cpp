@@ -172,17 +172,16 @@ extern "C" int rt_davidson_solve_dev(rt_matvec_fn a_x, void* ctx,
   const int64_t scr_n =
       std::max<int64_t>(dv_gram_scratch(n, cap, tmax), dv_gram_scratch(n, 1, 1) * tmax);
   RTA_TRY(scr.alloc((size_t)scr_n));
-  // small device area: e (cap), shift (cap), scale (cap), sq (cap), coef (cap*cap,
+  // small device area: e (cap), shift (cap), spare (cap), sq (cap), coef (cap*cap,
   // also the _qr coefficients), gram out (cap*tmax, also the _qr and projection Gram
   // matrices)
-  const size_t o_e = 0, o_shift = cap, o_scale = 2 * (size_t)cap,
+  const size_t o_e = 0, o_shift = cap,
                o_sq = 3 * (size_t)cap, o_coef = 4 * (size_t)cap,
                o_gram = o_coef + (size_t)cap * cap,
                o_end = o_gram + (size_t)cap * tmax;
   RTA_TRY(small.alloc(o_end));
   double* d_e = small.p + o_e;
   double* d_shift = small.p + o_shift;
-  double* d_scale = small.p + o_scale;
   double* d_sq = small.p + o_sq;
   double* d_coef = small.p + o_coef;
   double* d_gram = small.p + o_gram;
@@ -196,14 +195,13 @@ extern "C" int rt_davidson_solve_dev(rt_matvec_fn a_x, void* ctx,
   std::vector<double> heff, w, V, e, elast;
   std::vector<int> conv(nroots, 0);
   std::vector<double> dx_norm, de;
-  // pinned host area, laid out like the small device area: e, shift, scale,
+  // pinned host area, laid out like the small device area: e, shift, spare,
   // sq (cap each), coef (cap*cap), gram (cap*tmax), then the _qr Gram matrix
   // and coefficients (2*tmax*tmax)
   HostBuf hbuf;
   RTA_TRY(hbuf.alloc(o_end + 2 * (size_t)tmax * tmax));
   double* h_e = hbuf.p + o_e;
   double* h_shift = hbuf.p + o_shift;
-  double* h_scale = hbuf.p + o_scale;
   double* h_sq = hbuf.p + o_sq;
   double* h_coef = hbuf.p + o_coef;
   double* h_gram = hbuf.p + o_gram;
@@ -217,18 +215,30 @@ extern "C" int rt_davidson_solve_dev(rt_matvec_fn a_x, void* ctx,
     int xtlen = 0;
     if (fresh) {
       space = 0;
-      RTA_TRY(dv_qr(X0.p, nx0, n, lindep, XS.p, &xtlen, d_gram, d_coef, scr.p, h_qr, s));
+      RTA_TRY(dv_qr(X0.p, nx0, n, lindep, false, XS.p, &xtlen, d_gram, d_coef, scr.p, h_qr,
+                    s));
       if (xtlen == 0)
         return fail(RT_ERR_NUMERIC, "No more linear independent basis found");
       ++st.restarts;
-    } else if (nt > 1) {
-      RTA_TRY(dv_qr(T.p, nt, n, lindep, XS.p + (size_t)space * n, &xtlen, d_gram, d_coef,
-                    scr.p, h_qr, s));
-      xtlen = std::min(xtlen, 40);
     } else {
-      RTA_HIP(hipMemcpyAsync(XS.p + (size_t)space * n, T.p, sizeof(double) * n,
-                             hipMemcpyDeviceToDevice, s));
-      xtlen = nt;
+      // the projected corrections, not yet normalized: the lindep drop and
+      // normalization of davidson.rs:245-249 happen in the Gram matrix of
+      // _qr (for nt = 1, where the reference skips _qr, the same drop and
+      // normalization and nothing else)
+      RTA_TRY(dv_qr(T.p, nt, n, lindep, true, XS.p + (size_t)space * n, &xtlen, d_gram,
+                    d_coef, scr.p, h_qr, s));
+      xtlen = std::min(xtlen, 40);
+      if (xtlen == 0) {
+        // linear dependency in the trial space (davidson.rs:250-257)
+        bool ok = true;
+        for (int k = 0; k < nroots; ++k) {
+          if (k < (int)dx_norm.size())
+            conv[k] = conv[k] || dx_norm[k] < std::sqrt(tol);
+          ok = ok && conv[k];
+        }
+        status = ok ? RT_OK : RT_ERR_NOTCONV;
+        break;
+      }
     }
     if (space + xtlen > cap)
       return fail(RT_ERR_NUMERIC, "subspace overflow (%d + %d > %d)", space, xtlen, cap);
@@ -359,30 +369,22 @@ extern "C" int rt_davidson_solve_dev(rt_matvec_fn a_x, void* ctx,
                                hipMemcpyHostToDevice, s));
         RTA_TRY(dv_precond_diag(R.p, hdiag_dev, d_shift, nt, n, T.p, s));
       }
-      // normalize (on the device), project out the subspace, renormalize
-      // with lindep drops
+      // normalize (on the device), project out the subspace
       RTA_TRY(dv_normalize_cols(T.p, nt, n, d_sq, scr.p, s));
       RTA_TRY(dv_project_out_block(XS.p, n, space, T.p, nt, n, d_gram, scr.p, s));
+    }
+    const bool fresh_next = space + nroots > maxspace;
+    if (nt > 0 && fresh_next) {
+      // a fresh restart does not use the corrections; only the lindep test
+      // on them (davidson.rs:245-257) is needed here.  Otherwise the next
+      // _qr drops and normalizes them without a synchronization of its own.
       RTA_TRY(dv_sqnorms(T.p, nt, n, d_sq, scr.p, s));
       RTA_HIP(hipMemcpyAsync(h_sq, d_sq, sizeof(double) * nt,
                              hipMemcpyDeviceToHost, s));
       RTA_HIP(hipStreamSynchronize(s));
       int m = 0;
-      for (int j = 0; j < nt; ++j) {
-        if (h_sq[j] > lindep) {
-          if (m != j)
-            RTA_HIP(hipMemcpyAsync(T.p + (size_t)m * n, T.p + (size_t)j * n,
-                                   sizeof(double) * n, hipMemcpyDeviceToDevice, s));
-          h_scale[m] = 1.0 / std::sqrt(h_sq[j]);
-          ++m;
-        }
-      }
+      for (int j = 0; j < nt; ++j) m += h_sq[j] > lindep ? 1 : 0;
       nt = m;
-      if (nt > 0) {
-        RTA_HIP(hipMemcpyAsync(d_scale, h_scale, sizeof(double) * nt,
-                               hipMemcpyHostToDevice, s));
-        RTA_TRY(dv_scale_cols(T.p, d_scale, nt, n, s));
-      }
     }
     if (nt == 0) {
       // linear dependency in the trial space (davidson.rs:250-257)
@@ -395,7 +397,7 @@ extern "C" int rt_davidson_solve_dev(rt_matvec_fn a_x, void* ctx,
       status = ok ? RT_OK : RT_ERR_NOTCONV;
       break;
     }
-    fresh = space + nroots > maxspace;
+    fresh = fresh_next;
   }
   RTA_HIP(hipStreamSynchronize(s));
   for (int k = 0; k < nroots; ++k) conv_out[k] = conv[k];

@@ -261,14 +261,6 @@ int dv_precond_diag(const double* R, const double* hdiag, const double* shift,
   return RT_OK;
 }
 
-// X[:, k] *= scale[k]
-__global__ void scale_cols_kernel(double* __restrict__ X,
-                                  const double* __restrict__ scale, int64_t n) {
-  const int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  const int k = blockIdx.y;
-  if (x < n) X[x + k * n] *= scale[k];
-}
-
 // X[:, k] *= 1 / sqrt(sq[k])   (sq on the device)
 __global__ void rnorm_cols_kernel(double* __restrict__ X, const double* __restrict__ sq,
                                   int64_t n) {
@@ -286,16 +278,6 @@ int dv_normalize_cols(double* X, int nk, int64_t n, double* sq, double* scratch,
   RTA_HIP(hipGetLastError());
   return RT_OK;
 }
-
-int dv_scale_cols(double* X, const double* scale, int nk, int64_t n,
-                  hipStream_t s) {
-  if (nk <= 0) return RT_OK;
-  hipLaunchKernelGGL(scale_cols_kernel, dim3((unsigned)((n + 255) / 256), nk),
-                     dim3(256), 0, s, X, scale, n);
-  RTA_HIP(hipGetLastError());
-  return RT_OK;
-}
-
 
 // T[:, c] -= sum_s Q[:, s] G[s + ns*c]
 __global__ __launch_bounds__(256) void sub_combine_kernel(const double* __restrict__ Q,
@@ -362,8 +344,8 @@ static int gs_gram_host(const double* G, int k, double lindep, double* C,
   return nv;
 }
 
-int dv_qr(double* X, int nx, int64_t n, double lindep, double* Q, int* nout, double* G,
-          double* coef, double* scratch, double* host, hipStream_t s) {
+int dv_qr(double* X, int nx, int64_t n, double lindep, bool prescale, double* Q, int* nout,
+          double* G, double* coef, double* scratch, double* host, hipStream_t s) {
   KernelTimer kt("dv_mgs", s);
   *nout = 0;
   int k = nx;
@@ -383,7 +365,34 @@ int dv_qr(double* X, int nx, int64_t n, double lindep, double* Q, int* nout, dou
     RTA_HIP(hipStreamSynchronize(s));
     // the synchronization above also retired the previous pass's upload and
     // combine, so host and coef are free again
-    const int nv = gs_gram_host(Gh, k, lindep, Ch, &min_frac);
+    int nv;
+    if (pass == 0 && prescale) {
+      // unit-normalize first: column i is dropped when G_ii <= lindep, the
+      // others enter as x_i / |x_i| (G_ab -> G_ab / (|x_a| |x_b|)); C is
+      // scaled back to the unnormalized columns, zero rows for the dropped
+      std::vector<int> idx;
+      std::vector<double> sc;
+      for (int i = 0; i < k; ++i) {
+        const double gii = Gh[i + (size_t)k * i];
+        if (gii > lindep) {
+          idx.push_back(i);
+          sc.push_back(1.0 / std::sqrt(gii));
+        }
+      }
+      const int m = (int)idx.size();
+      std::vector<double> Gs((size_t)m * m), Cs((size_t)m * std::max(m, 1));
+      for (int b = 0; b < m; ++b)
+        for (int a = 0; a < m; ++a)
+          Gs[a + (size_t)m * b] = sc[a] * Gh[idx[a] + (size_t)k * idx[b]] * sc[b];
+      nv = m > 0 ? gs_gram_host(Gs.data(), m, lindep, Cs.data(), &min_frac) : 0;
+      for (int j = 0; j < nv; ++j) {
+        double* cj = Ch + (size_t)k * j;
+        for (int r = 0; r < k; ++r) cj[r] = 0.0;
+        for (int a = 0; a < m; ++a) cj[idx[a]] = sc[a] * Cs[a + (size_t)m * j];
+      }
+    } else {
+      nv = gs_gram_host(Gh, k, lindep, Ch, &min_frac);
+    }
     if (nv > 0) {
       RTA_HIP(hipMemcpyAsync(coef, Ch, sizeof(double) * k * nv, hipMemcpyHostToDevice, s));
       RTA_TRY(dv_combine(src, n, k, coef, k, nv, n, dst, n, s));

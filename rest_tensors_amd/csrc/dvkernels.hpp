@@ -58,8 +58,6 @@ int dv_sqnorms(const double* X, int nk, int64_t n, double* sq_out,
                double* scratch, hipStream_t s);
 int dv_precond_diag(const double* R, const double* hdiag, const double* shift,
                     int nk, int64_t n, double* T, hipStream_t s);
-int dv_scale_cols(double* X, const double* scale, int nk, int64_t n,
-                  hipStream_t s);
 // X[:, k] /= |X[:, k]| for k < nk, the norms computed and applied on the
 // device (sq: nk doubles of device output; scratch as dv_sqnorms)
 int dv_normalize_cols(double* X, int nk, int64_t n, double* sq, double* scratch,
@@ -74,8 +72,11 @@ int dv_project_out_block(const double* Q, int64_t ldq, int ns, double* T, int nt
 // *nout (host).  X is overwritten.  G, coef: device, >= nx^2 doubles;
 // scratch >= dv_gram_scratch(n, nx, nx); host: >= 2 nx^2 doubles (the Gram
 // matrix and the coefficients; pinned memory keeps the two transfers short)
-// that stay valid until the stream has passed this call.
-int dv_qr(double* X, int nx, int64_t n, double lindep, double* Q, int* nout, double* G,
-          double* coef, double* scratch, double* host, hipStream_t s);
+// that stay valid until the stream has passed this call.  prescale: the
+// columns of X are not normalized yet; one with squared norm <= lindep is
+// dropped and the others are normalized before the Gram-Schmidt (all in the
+// Gram matrix).
+int dv_qr(double* X, int nx, int64_t n, double lindep, bool prescale, double* Q, int* nout,
+          double* G, double* coef, double* scratch, double* host, hipStream_t s);
 
 }  // namespace rta
