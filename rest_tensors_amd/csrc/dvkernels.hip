@@ -44,10 +44,14 @@ __device__ double dv_block_sum(double v, double* red) {
 }
 
 // ---- Gram blocks: G[i + ldg*j] = U[:, i] . W[:, j] ---------------------------
-// Block (slice of rows, 8 columns of U, 8 columns of W): every
-// thread sums its rows into an 8 x 8 register tile, the tiles are summed per
-// wave and across the 4 waves in a fixed order; partial[s][i + nu*j] per
-// slice s, then a fixed-order sum over the slices.
+// Block (slice of rows, 8 columns of U, 8 columns of W): every thread sums its
+// rows into an 8 x 8 register tile; the 64 values are reduce-scattered across
+// the wave (6 butterfly steps that halve the values each lane holds: lane l
+// ends with the wave's sum of value l, 63 shuffles instead of 64 x 6; 1024-row
+// slices measured faster than 256-row ones, whose slice sum costs more than
+// the shorter row loop saves), then summed across the 4 waves in a fixed
+// order; partial[s][i + nu*j] per slice s, then a fixed-order sum over the
+// slices.
 // rows per slice: at least kGramRows, at most kGramSlices slices
 constexpr int64_t kGramRows = 1024, kGramSlices = 512;
 
@@ -55,17 +59,15 @@ __global__ __launch_bounds__(256) void gram_tile_kernel(
     const double* __restrict__ U, int64_t ldu, int nu,
     const double* __restrict__ W, int64_t ldw, int nw, int64_t n, int64_t slice,
     double* __restrict__ partial) {
-  constexpr int TI = 8, TJ = 8;
-  __shared__ double red[4][TI * TJ];
+  constexpr int TI = 8, TJ = 8, NV = TI * TJ;
+  __shared__ double red[4][NV];
   const int64_t x0 = (int64_t)blockIdx.x * slice;
   const int64_t x1 = (x0 + slice < n) ? x0 + slice : n;
   const int i0 = blockIdx.y * TI, j0 = blockIdx.z * TJ;
   const int ni = (nu - i0 < TI) ? nu - i0 : TI, nj = (nw - j0 < TJ) ? nw - j0 : TJ;
-  double acc[TI][TJ];
+  double v[NV];  // v[i + TI j]
 #pragma unroll
-  for (int i = 0; i < TI; ++i)
-#pragma unroll
-    for (int j = 0; j < TJ; ++j) acc[i][j] = 0.0;
+  for (int q = 0; q < NV; ++q) v[q] = 0.0;
   for (int64_t x = x0 + threadIdx.x; x < x1; x += 256) {
     double u[TI], w[TJ];
 #pragma unroll
@@ -75,19 +77,25 @@ __global__ __launch_bounds__(256) void gram_tile_kernel(
 #pragma unroll
     for (int i = 0; i < TI; ++i)
 #pragma unroll
-      for (int j = 0; j < TJ; ++j) acc[i][j] += u[i] * w[j];
+      for (int j = 0; j < TJ; ++j) v[i + TI * j] += u[i] * w[j];
   }
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  // step with offset o: a lane with bit o set keeps the upper half of its
+  // values and sends the lower half to its partner, which does the opposite
 #pragma unroll
-  for (int i = 0; i < TI; ++i)
+  for (int o = 32, h = NV / 2; o >= 1; o >>= 1, h >>= 1) {
+    const bool up = (lane & o) != 0;
 #pragma unroll
-    for (int j = 0; j < TJ; ++j) {
-      const double v = dv_wave_sum(acc[i][j]);
-      if (lane == 0) red[wave][i + TI * j] = v;
+    for (int q = 0; q < h; ++q) {
+      const double send = up ? v[q] : v[q + h];
+      const double keep = up ? v[q + h] : v[q];
+      v[q] = keep + __shfl_xor(send, o, 64);
     }
+  }
+  red[wave][lane] = v[0];
   __syncthreads();
   const int t = threadIdx.x;
-  if (t < TI * TJ) {
+  if (t < NV) {
     const int i = t % TI, j = t / TI;
     if (i < ni && j < nj)
       partial[(int64_t)blockIdx.x * nu * nw + (i0 + i) + (int64_t)nu * (j0 + j)] =
