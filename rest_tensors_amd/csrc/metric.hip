@@ -36,9 +36,11 @@ __device__ __forceinline__ void panel_row(const double* __restrict__ M, int64_t 
 
 // One workgroup of 64 threads factors W[k0:k0+kb, k0:k0+kb] (lower),
 // right-looking: thread t keeps row t in registers; at step j thread j takes
-// the pivot, every row below scales its entry j, the column is shared
-// through LDS and each row updates its entries j < c <= t.  The steps are
-// unrolled, so the row stays in registers.  info (device): first failing
+// the pivot, every row below scales its entry j by the pivot's reciprocal (as
+// LAPACK's DPOTF2 does), the column is shared through LDS and each row updates
+// its entries c > j, all of them: the entries c > t (the upper triangle)
+// take garbage that is never stored, which saves a divergent branch per
+// entry.  The steps are unrolled, so the row stays in registers.  info (device): first failing
 // pivot (1-based) if still 0.
 __global__ __launch_bounds__(CH_NB) void potf2_kernel(double* __restrict__ W, int64_t ld,
                                                       int64_t k0, int kb,
@@ -59,16 +61,16 @@ __global__ __launch_bounds__(CH_NB) void potf2_kernel(double* __restrict__ W, in
       const double d = r[j];
       if (j < kb && !(d > 0.0) && *info == 0) *info = (int)(k0 + j + 1);
       r[j] = sqrt(d);
-      piv = r[j];
+      piv = 1.0 / r[j];
     }
     __syncthreads();
-    if (t > j) r[j] /= piv;
-    col[t] = r[j];
+    if (t > j) r[j] *= piv;
+    const double lj = r[j];
+    col[t] = lj;
     __syncthreads();
     if (t > j) {
 #pragma unroll
-      for (int c = j + 1; c < CH_NB; ++c)
-        if (c <= t) r[c] -= r[j] * col[c];
+      for (int c = j + 1; c < CH_NB; ++c) r[c] -= lj * col[c];
     }
     __syncthreads();  // col and piv are rewritten by the next step
   }
