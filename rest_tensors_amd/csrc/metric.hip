@@ -153,20 +153,34 @@ __global__ void mt_fill_kernel(double* __restrict__ X, int64_t len, double v) {
 // W (ld n) <- lower Cholesky factor of W in its lower triangle (the strict
 // upper triangle is left holding intermediate values); info (host) = first
 // non-positive pivot (1-based) or 0.
+// Two levels of blocking: 64-column panels (potf2 + trsm_rows) inside
+// 256-column blocks; a panel's update reaches only the rest of its block, and
+// each block ends with one rank-256 update of the trailing matrix (a GEMM of
+// depth 256 instead of four of depth 64, which re-read and re-write the
+// trailing matrix four times).
 int chol_lower(double* W, int64_t n, int* info_h, int* info_d, hipStream_t s) {
+  constexpr int64_t NB2 = 4 * CH_NB;
   RTA_HIP(hipMemsetAsync(info_d, 0, sizeof(int), s));
-  for (int64_t k0 = 0; k0 < n; k0 += CH_NB) {
-    const int kb = (int)std::min<int64_t>(CH_NB, n - k0);
-    hipLaunchKernelGGL(potf2_kernel, dim3(1), dim3(CH_NB), 0, s, W, n, k0, kb, info_d);
-    const int64_t r0 = k0 + kb;
-    if (r0 < n) {
-      hipLaunchKernelGGL(trsm_rows_kernel, dim3((unsigned)((n - r0 + CH_NB - 1) / CH_NB)),
-                         dim3(CH_NB), 0, s, W, n, k0, kb, r0, n);
-      const int64_t m2 = n - r0;
-      RTA_TRY(rt_dgemm_dev('N', 'T', m2, m2, kb, -1.0, W + r0 + n * k0, n, W + r0 + n * k0, n,
-                           1.0, W + r0 + n * r0, n, s));
+  for (int64_t b0 = 0; b0 < n; b0 += NB2) {
+    const int64_t b1 = std::min<int64_t>(b0 + NB2, n);
+    for (int64_t k0 = b0; k0 < b1; k0 += CH_NB) {
+      const int kb = (int)std::min<int64_t>(CH_NB, b1 - k0);
+      hipLaunchKernelGGL(potf2_kernel, dim3(1), dim3(CH_NB), 0, s, W, n, k0, kb, info_d);
+      const int64_t r0 = k0 + kb;
+      if (r0 < n) {
+        hipLaunchKernelGGL(trsm_rows_kernel, dim3((unsigned)((n - r0 + CH_NB - 1) / CH_NB)),
+                           dim3(CH_NB), 0, s, W, n, k0, kb, r0, n);
+        if (r0 < b1)  // the block's remaining columns, all rows below
+          RTA_TRY(rt_dgemm_dev('N', 'T', n - r0, b1 - r0, kb, -1.0, W + r0 + n * k0, n,
+                               W + r0 + n * k0, n, 1.0, W + r0 + n * r0, n, s));
+      }
+      RTA_HIP(hipGetLastError());
     }
-    RTA_HIP(hipGetLastError());
+    if (b1 < n) {
+      const int64_t m2 = n - b1;
+      RTA_TRY(rt_dgemm_dev('N', 'T', m2, m2, b1 - b0, -1.0, W + b1 + n * b0, n, W + b1 + n * b0,
+                           n, 1.0, W + b1 + n * b1, n, s));
+    }
   }
   RTA_HIP(hipMemcpyAsync(info_h, info_d, sizeof(int), hipMemcpyDeviceToHost, s));
   RTA_HIP(hipStreamSynchronize(s));
