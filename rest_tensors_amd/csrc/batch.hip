@@ -174,6 +174,59 @@ __global__ __launch_bounds__(256) void batch_j_kernel(const MolDev* __restrict__
   if (i != j) M.J[j + i * nb] = v;
 }
 
+// TN path: the same sums over the padded copy Ap (columns 16-byte aligned,
+// ld nbp), one lane per pair of rows (i, i+1), i even, of column j's upper
+// part with 16-byte loads; column j holds floor(j/2) + 1 row pairs, so the
+// pairs of columns < j number j^2/4 rounded up.  Per element the sum runs in
+// the order of batch_j_kernel (results bit-identical).
+__device__ __forceinline__ int64_t bj_pairs_before(int64_t j) {
+  return (j / 2) * (j / 2 + 1) + (j & 1) * (j / 2 + 1);
+}
+
+__global__ __launch_bounds__(256) void batch_j_x2_kernel(const MolDev* __restrict__ mols,
+                                                         const int64_t* __restrict__ offs,
+                                                         int nmol) {
+  constexpr int U = 16;
+  const int m = find_mol(offs, nmol, blockIdx.x);
+  const MolDev M = mols[m];
+  const int64_t nb = M.nb, nbp = M.nbp, n2h = nbp * nbp / 2, naux = M.naux;
+  const int64_t q = (blockIdx.x - offs[m]) * 256 + threadIdx.x;
+  if (q >= bj_pairs_before(nb)) return;
+  int64_t j = (int64_t)(2.0 * sqrt((double)q));
+  while (j > 0 && bj_pairs_before(j) > q) --j;
+  while (bj_pairs_before(j + 1) <= q) ++j;
+  const int64_t i = 2 * (q - bj_pairs_before(j));
+  const double2* p = reinterpret_cast<const double2*>(M.Ap + i + j * nbp);
+  const double* gm = M.gamma;
+  double accx = 0.0, accx2 = 0.0, accy = 0.0, accy2 = 0.0;
+  int64_t P = 0;
+  for (; P + U - 1 < naux; P += U) {
+    double2 a[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) a[u] = p[(P + u) * n2h];
+#pragma unroll
+    for (int u = 0; u < U; u += 2) {
+      accx += a[u].x * gm[P + u];
+      accx2 += a[u + 1].x * gm[P + u + 1];
+      accy += a[u].y * gm[P + u];
+      accy2 += a[u + 1].y * gm[P + u + 1];
+    }
+  }
+  for (; P < naux; ++P) {
+    const double2 a = p[P * n2h];
+    accx += a.x * gm[P];
+    accy += a.y * gm[P];
+  }
+  const double vx = accx + accx2;
+  M.J[i + j * nb] = vx;
+  if (i != j) M.J[j + i * nb] = vx;
+  if (i + 1 <= j) {
+    const double vy = accy + accy2;
+    M.J[(i + 1) + j * nb] = vy;
+    if (i + 1 != j) M.J[j + (i + 1) * nb] = vy;
+  }
+}
+
 // one block per (molecule, upper 32x32 tile): K = sum of partial slots, mirrored
 __global__ __launch_bounds__(256) void batch_kreduce_kernel(
     const MolDev* __restrict__ mols, const int64_t* __restrict__ offs, int nmol) {
@@ -336,6 +389,8 @@ struct Batch {
   std::vector<MolDev> mols;
   MolDev* d_mols = nullptr;
   int64_t *d_off_gamma = nullptr, *d_off_j = nullptr, *d_off_kred = nullptr;
+  int64_t* d_off_j2 = nullptr;  // (molecule, 256 row pairs) for batch_j_x2_kernel
+  int64_t n_j2 = 0;
   int64_t n_gamma = 0, n_j = 0, n_kred = 0;
   double* energies = nullptr;  // device [2*nmol]
   std::vector<void*> owned;    // device allocations of the handle
@@ -660,7 +715,10 @@ static int run_gamma_j(Batch* b, hipStream_t sj) {
   else if (!b->use_tn && b->n_gpair > 0)
     hipLaunchKernelGGL(batch_gamma_kernel, dim3((unsigned)b->n_gpair), dim3(256), 0, sj,
                        b->d_mols, b->d_off_gpair, b->nmol);
-  if (b->n_j > 0)
+  if (b->use_tn && b->n_j2 > 0)
+    hipLaunchKernelGGL(batch_j_x2_kernel, dim3((unsigned)b->n_j2), dim3(256), 0, sj, b->d_mols,
+                       b->d_off_j2, b->nmol);
+  else if (b->n_j > 0)
     hipLaunchKernelGGL(batch_j_kernel, dim3((unsigned)b->n_j), dim3(256), 0, sj, b->d_mols,
                        b->d_off_j, b->nmol);
   RTA_HIP(hipGetLastError());
@@ -750,9 +808,9 @@ extern "C" int rt_batch_create(const rt_mol_desc* mols, int nmol, void** handle)
   const char* forced = getenv("RTA_BATCH_GROUPED");
   const bool use_tn = !(forced && atoi(forced) == 1);
   b->use_tn = use_tn;
-  int64_t ng = 0, nj = 0, nk = 0, nc = 0, nbc = 0, ngp = 0, ngd = 0;
+  int64_t ng = 0, nj = 0, nk = 0, nc = 0, nbc = 0, ngp = 0, ngd = 0, nj2 = 0;
   std::vector<int64_t> og(1, 0), oj(1, 0), ok(1, 0), oc(1, 0), obc(1, 0), ogp(1, 0),
-      ogd(1, 0);
+      ogd(1, 0), oj2(1, 0);
   for (int i = 0; i < nmol; ++i) {
     const rt_mol_desc& m = mols[i];
     if (!(m.nbas > 0 && m.naux > 0 && m.nocc > 0 && m.nocc <= m.nbas && m.ri3ao && m.C &&
@@ -799,6 +857,11 @@ extern "C" int rt_batch_create(const rt_mol_desc* mols, int nmol, void** handle)
     const int64_t t32 = (m.nbas + 31) / 32;
     ng += m.naux;
     nj += (m.nbas * (m.nbas + 1) / 2 + 255) / 256;
+    {
+      const int64_t h = m.nbas / 2, npairs = h * (h + 1) + (m.nbas & 1) * (h + 1);
+      nj2 += (npairs + 255) / 256;
+      oj2.push_back(nj2);
+    }
     nk += t32 * (t32 + 1) / 2;
     nc += m.nbas;
     nbc += m.nbas * ((m.naux + 31) / 32);
@@ -814,6 +877,7 @@ extern "C" int rt_batch_create(const rt_mol_desc* mols, int nmol, void** handle)
   }
   b->n_gamma = ng;
   b->n_j = nj;
+  b->n_j2 = nj2;
   b->n_kred = nk;
   b->n_cpad = nc;
   b->n_bcopy = nbc;
@@ -823,6 +887,7 @@ extern "C" int rt_batch_create(const rt_mol_desc* mols, int nmol, void** handle)
   int st = dalloc(b, sizeof(MolDev) * nmol, (void**)&b->d_mols);
   if (st == RT_OK) st = dalloc(b, sizeof(int64_t) * og.size(), (void**)&b->d_off_gamma);
   if (st == RT_OK) st = dalloc(b, sizeof(int64_t) * oj.size(), (void**)&b->d_off_j);
+  if (st == RT_OK) st = dalloc(b, sizeof(int64_t) * oj2.size(), (void**)&b->d_off_j2);
   if (st == RT_OK) st = dalloc(b, sizeof(int64_t) * ok.size(), (void**)&b->d_off_kred);
   if (st == RT_OK) st = dalloc(b, sizeof(int64_t) * oc.size(), (void**)&b->d_off_cpad);
   if (st == RT_OK) st = dalloc(b, sizeof(int64_t) * obc.size(), (void**)&b->d_off_bcopy);
@@ -836,6 +901,8 @@ extern "C" int rt_batch_create(const rt_mol_desc* mols, int nmol, void** handle)
                 hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(b->d_off_j, oj.data(), sizeof(int64_t) * oj.size(), hipMemcpyHostToDevice) !=
           hipSuccess ||
+      hipMemcpy(b->d_off_j2, oj2.data(), sizeof(int64_t) * oj2.size(),
+                hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(b->d_off_kred, ok.data(), sizeof(int64_t) * ok.size(),
                 hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(b->d_off_cpad, oc.data(), sizeof(int64_t) * oc.size(),
