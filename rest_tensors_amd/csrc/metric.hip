@@ -1,7 +1,7 @@
 // FP64 aux-metric functions kept in FP64 on the RI path (A11):
 //   _dpotrf                     src/matrix/matrix_blas_lapack.rs:518-534
 //   cholesky_decompose_inverse  src/matrix/matrix_blas_lapack.rs:931-1002
-// Blocked right-looking Cholesky with 64-wide panels: the diagonal block is
+// Blocked right-looking Cholesky with CH_NB-wide panels: the diagonal block is
 // factored by one workgroup in LDS, the panel below by a row-parallel
 // triangular solve, the trailing matrix by the FP64 MFMA GEMM.  Inverses go
 // through the triangular inverse (blocked forward substitution) and one TN
@@ -19,11 +19,14 @@
 
 namespace rta {
 
-constexpr int CH_NB = 64;
+// Panel width: the panel kernels are one chain of dependent steps per row, so
+// their latency (not the GPU's throughput) sets the pace; 32-column panels
+// halve each chain (the shift-invert setup at m = 2048: 8.6 -> 7.2-8.1 ms).
+constexpr int CH_NB = 32;
 
 static unsigned mt_grid(int64_t len) { return (unsigned)((len + 255) / 256); }
 
-// Row r0 + t of the 64-column block of M (ld) at columns c0.. into out[64]:
+// Row r0 + t of the CH_NB-column block of M (ld) at columns c0.. into out[CH_NB]:
 // every address is clamped into the nr x nc valid part so that all 64 loads
 // are issued before the first is waited for (a bounds branch per load would
 // serialize them); values outside the valid part are replaced by the caller.
@@ -34,7 +37,7 @@ __device__ __forceinline__ void panel_row(const double* __restrict__ M, int64_t 
   for (int j = 0; j < CH_NB; ++j) out[j] = row[ld * (c0 + (j < nc ? j : nc - 1))];
 }
 
-// One workgroup of 64 threads factors W[k0:k0+kb, k0:k0+kb] (lower),
+// One workgroup of CH_NB threads factors W[k0:k0+kb, k0:k0+kb] (lower),
 // right-looking: thread t keeps row t in registers; at step j thread j takes
 // the pivot, every row below scales its entry j by the pivot's reciprocal (as
 // LAPACK's DPOTF2 does), the column is shared through LDS and each row updates
@@ -82,7 +85,7 @@ __global__ __launch_bounds__(CH_NB) void potf2_kernel(double* __restrict__ W, in
 
 // Rows r in [r0, r1): W[r, k0:k0+kb] <- W[r, k0:k0+kb] L11^-T (L11 = the
 // factored diagonal block), i.e. x_j = (a_j - sum_{i<j} x_i L11[j][i]) / L11[j][j].
-// 64 threads, one row each; the block's rows and L11 live in LDS.
+// CH_NB threads, one row each; the block's rows and L11 live in LDS.
 __global__ __launch_bounds__(CH_NB) void trsm_rows_kernel(double* __restrict__ W, int64_t ld,
                                                           int64_t k0, int kb, int64_t r0,
                                                           int64_t r1) {
@@ -153,14 +156,13 @@ __global__ void mt_fill_kernel(double* __restrict__ X, int64_t len, double v) {
 // W (ld n) <- lower Cholesky factor of W in its lower triangle (the strict
 // upper triangle is left holding intermediate values); info (host) = first
 // non-positive pivot (1-based) or 0.
-// Two levels of blocking: 64-column panels (potf2 + trsm_rows) inside
+// Two levels of blocking: CH_NB-column panels (potf2 + trsm_rows) inside
 // 512-column blocks; a panel's update reaches only the rest of its block, and
-// each block ends with one rank-512 update of the trailing matrix (a GEMM of
-// depth 512 instead of eight of depth 64, which re-read and re-write the
-// trailing matrix eight times; at n = 4096 blocks of 256 measured 2-4 %
-// slower).
+// each block ends with one rank-512 update of the trailing matrix (one GEMM of
+// depth 512 instead of one per panel, each re-reading and re-writing the
+// trailing matrix; at n = 4096 blocks of 256 measured 2-4 % slower).
 int chol_lower(double* W, int64_t n, int* info_h, int* info_d, hipStream_t s) {
-  constexpr int64_t NB2 = 8 * CH_NB;
+  constexpr int64_t NB2 = 512;
   RTA_HIP(hipMemsetAsync(info_d, 0, sizeof(int), s));
   for (int64_t b0 = 0; b0 < n; b0 += NB2) {
     const int64_t b1 = std::min<int64_t>(b0 + NB2, n);
